@@ -1,0 +1,188 @@
+"""Benchmark: N-Queens counting on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 21] [--rows 6]
+
+A step is one full count of the N=21 board with 6 rows pre-placed (4,470,757 prefix tasks,
+the reference's single-GPU configuration, README.md:24,28), with the task list already
+resident in HBM.  The value is DFS nodes per second over the timed steps (a node is one queen
+placed at depths rows+1..N-1).  Also reported:
+  * n21_wall_s: the whole `n_queens 21 6` CLI call (generation, device init, copies, kernel) —
+    the reference's own measure (src/main.cpp:49-54; 21.242 s on one RTX 5090, README.md:24);
+  * roofline: integer-VALU bound of the DFS kernel (11 int32 ops per node);
+  * cpu_baseline: the reference's own OpenMP path (oracle/_ref, parallel_n_queens) timed on this
+    host on a bounded sample.
+Prints ONE JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "N=21 wall s + DFS nodes/s on 1 GPU; batched-mix instances/s vs per-launch"
+OPS_PER_NODE = 11          # p=v&-v (2), v^=p (1), c|=p (1), l=(l|p)<<1 (2), r=(r|p)>>1 (2), v'=last&~(c|l|r) (3)
+PEAK_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
+LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(nodes_fn) -> dict | None:
+    """The reference's OpenMP path (32 threads, src/n_queens.cpp:139-163) on N=18 rows 6."""
+    import oracle
+    R = oracle.ref()
+    if R is None:
+        return None
+    N, rows = 18, 6
+    t0 = time.perf_counter()
+    total = int(R.ref_parallel_n_queens(N, rows))
+    dt = time.perf_counter() - t0
+    nodes = nodes_fn(N, rows)
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": nodes / dt, "unit": "nodes/s", "cores": 32, "kind": "reference",
+            "sample": f"whole N={N} board, {rows} rows pre-placed, reference parallel_n_queens "
+                      f"(omp_set_num_threads(32)) in {dt:.2f} s, {total} solutions, {nodes} nodes; "
+                      f"host: {ncpu} usable CPUs, {model}",
+            "seconds": dt, "solutions": total}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=21)
+    ap.add_argument("--rows", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-wall", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
+              f"path (asked for --gpus {args.gpus}).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import n_queens_amd as nq
+
+    N, rows = args.n, args.rows
+    info = nq.device_info(0)
+    log(f"device {info}")
+    t0 = time.perf_counter()
+    tasks = nq.generate(N, rows)
+    gen_s = time.perf_counter() - t0
+    ntasks = len(tasks)
+    log(f"generated {ntasks} tasks for N={N} rows={rows} in {gen_s*1e3:.1f} ms")
+
+    dev = torch.device("cuda:0")
+    d_tasks = torch.from_numpy(tasks.view(np.int32).reshape(-1)).to(dev)
+    d_result = torch.zeros(2, dtype=torch.int64, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.synchronize()
+    cfg = nq.config_default(N, rows)
+
+    def run(count_nodes: bool) -> None:
+        nq.count_device(N, d_tasks.data_ptr(), ntasks, d_result.data_ptr(), cfg=cfg, count_nodes=count_nodes,
+                        stream=stream.cuda_stream)
+
+    # Node count (node-counting kernel, once; also the first warmup).
+    d_result.zero_()
+    torch.cuda.synchronize()
+    run(True)
+    stream.synchronize()
+    res = d_result.cpu().numpy().view(np.uint64)
+    sum0, nodes = int(res[0]), int(res[1])
+    log(f"node-counting run: sum={sum0} total={2*sum0} nodes={nodes}")
+    for i in range(max(0, args.warmup - 1)):
+        d_result.zero_()
+        run(False)
+        stream.synchronize()
+        log(f"warmup {i+1} done")
+
+    # Timed steps: events on the launch stream.
+    d_result.zero_()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        run(False)
+        ev[i][1].record(stream)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    step_ms = [a.elapsed_time(b) for a, b in ev]
+    res = d_result.cpu().numpy().view(np.uint64)
+    sums = int(res[0])
+    ok = sums == args.steps * sum0
+    log(f"timed steps ms={['%.1f' % x for x in step_ms]} wall={wall:.3f}s counts_consistent={ok}")
+    if not ok:
+        log("ERROR: counts differ between steps")
+        return 1
+    ms_per_step = sum(step_ms) / len(step_ms)
+    nodes_per_s = nodes / (ms_per_step / 1e3)
+    peak = info["cus"] * LANES_PER_CU * PEAK_CLOCK_HZ / 1e12
+    achieved = nodes * OPS_PER_NODE / (ms_per_step / 1e3) / 1e12
+
+    out = {
+        "metric": METRIC, "value": nodes_per_s, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "strong",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"N={N} whole-board count, {rows} rows pre-placed (configuration2)", "N": N,
+                   "rows": rows, "ntasks": ntasks, "block": cfg.block, "stack_depth": cfg.stack_depth,
+                   "chunk": cfg.chunk, "parallelism": "single GPU, persistent grid"},
+        "solutions": 2 * sum0, "nodes": nodes,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tops/s (int32 lane-ops)",
+                     "frac": achieved / peak, "traffic": None,
+                     "basis": f"{OPS_PER_NODE} int32 ops per DFS node x {nodes} nodes / kernel time; peak = "
+                              f"{info['cus']} CUs x 64 lanes x 2.4 GHz"},
+    }
+
+    if not args.no_wall:
+        t0 = time.perf_counter()
+        p = subprocess.run([nq.CLI_PATH, str(N), str(rows)], capture_output=True, text=True, timeout=900)
+        outer = time.perf_counter() - t0
+        m = re.search(r"cuda (\d+) queens result (-?\d+), calc time: \[([\d.]+)ms\]", p.stdout)
+        if m:
+            out["n21_wall_s" if N == 21 else "wall_s"] = float(m.group(3)) / 1e3
+            out["wall_process_s"] = outer
+            out["wall_result_ok"] = int(m.group(2)) == 2 * sum0
+            out["wall_vs_rtx5090"] = (21.242 / (float(m.group(3)) / 1e3)) if N == 21 else None
+            log(f"CLI: {m.group(0)} (process {outer:.2f} s)")
+        else:
+            log(f"CLI output not parsed: {p.stdout[-500:]} {p.stderr[-500:]}")
+
+    if not args.no_cpu:
+        def nodes_fn(n, r):
+            t = nq.generate(n, r)
+            return nq.count(n, t, nodes=True).nodes
+        cb = cpu_baseline(nodes_fn)
+        if cb:
+            out["cpu_baseline"] = cb
+            out["cpu_est_wall_s"] = nodes / cb["value"]
+            log(f"cpu baseline {cb['value']:.3e} nodes/s ({cb['seconds']:.1f} s)")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

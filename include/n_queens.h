@@ -1,0 +1,29 @@
+// n_queens.h — C++ drop-in for the reference's GPU counting path.
+//
+// Same names, signatures and C++ linkage as the reference header src/n_queens.h:7-19 for the
+// functions on the GPU path, so that a main.cpp-style caller (src/main.cpp:51) links unchanged
+// against libnq_hip.so.  cuda_n_queens keeps its name as "the GPU count" even though it runs
+// HIP on an MI355X; it is implemented on top of the C ABI in nq.h.
+//
+// The reference's CPU-only counters (n_queens, n_queens_iterative, serial_n_queens,
+// parallel_n_queens; src/n_queens.h:7-17) are not part of this GPU library.
+#pragma once
+
+#include <vector>
+
+using namespace std;  // as the reference header does (src/n_queens.h:5)
+
+// partial_n_queens (src/n_queens.cpp:74-91): append the (cur, left, right) triple of every
+// placement of `rows` more queens below (cur, left, right) to tot; row 0 (cur == 0) is limited
+// to columns < N/2.
+void partial_n_queens(int N, int cur, int left, int right, vector<int> &tot, int rows);
+
+// partial_n_queens_for_odd (src/n_queens.cpp:93-113): the same with row 0 limited to the middle
+// column and the row after a single queen limited to columns < (N-2)/2.
+void partial_n_queens_for_odd(int N, int cur, int left, int right, vector<int> &tot, int rows);
+
+// cuda_n_queens (src/n_queens_cuda.cu:84-186): the number of N-Queens solutions, counted on the
+// GPU with the first `rows` rows pre-placed.  Prints the reference's progress lines.  Returns -1
+// when no GPU is present (src/n_queens_cuda.cu:105-108) and, unlike the reference (which calls
+// exit(-1), src/macro.h:40-49), also -1 after any other failure, with the reason on stderr.
+long long cuda_n_queens(int N, int rows);

@@ -1,0 +1,110 @@
+/*
+ * nq.h — C ABI of libnq_hip.so, the MI355X (gfx950) N-Queens solution counter.
+ *
+ * Plain C: integers, pointers and sizes only.  Every function returns NQ_OK (0) or a negative
+ * nq_status and never exits the process; nq_last_error() gives the message of the last failure
+ * on the calling thread.  Host arrays are owned by the caller; device memory is owned by the
+ * library unless a function says it takes device pointers.
+ *
+ * The reference (ygch/n_queens) has no FFI: its only boundary is the C++ header
+ * src/n_queens.h:7-19, called from src/main.cpp:51.  include/n_queens.h keeps those C++ names
+ * on top of this ABI; each entry point below cites the reference code it replaces.
+ */
+#ifndef NQ_H
+#define NQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nq_status {
+    NQ_OK = 0,
+    NQ_ERR_ARG = -1,      /* invalid argument (N, rows, sizes, null pointers)            */
+    NQ_ERR_NO_DEVICE = -2,/* no HIP device visible (reference: n_queens_cuda.cu:105-108)   */
+    NQ_ERR_HIP = -3,      /* a HIP runtime call failed (reference: CU_SAFE_CALL, macro.h)  */
+    NQ_ERR_DEPTH = -4,    /* stack depth too small for (N, prefix depth)                   */
+    NQ_ERR_NOMEM = -5,    /* host or device allocation failed                              */
+    NQ_ERR_BUSY = -6,     /* admission refused (serving: task or LDS budget exhausted)     */
+    NQ_ERR_STATE = -7     /* call not valid in the handle's current state                  */
+};
+
+/* Launch configuration of the DFS kernel.  Zero fields take the defaults chosen by
+ * nq_config_default().  The reference fixes these at compile time (src/macro.h:25-35:
+ * CONFIG1 128 threads / depth 24, CONFIG2 160 / 19, CONFIG3 192 / 16) plus grid 1024
+ * (src/n_queens_cuda.cu:164-167). */
+typedef struct nq_config {
+    int block;        /* threads per workgroup, a multiple of 64, <= 256                  */
+    int stack_depth;  /* LDS frames per lane; must be >= N - 1 - (prefix depth)           */
+    int waves_per_cu; /* resident-wave target that sizes the persistent grid (0 = max)     */
+    int chunk;        /* tasks a wave claims per cursor grab (1..64)                      */
+    int grid;         /* explicit workgroup count (0 = from waves_per_cu)                  */
+} nq_config;
+
+/* Per-call statistics (all optional). */
+typedef struct nq_stats {
+    uint64_t ntasks;      /* tasks counted                                              */
+    uint64_t nodes;       /* DFS nodes (only when requested; else 0)                    */
+    double gen_ms;        /* host prefix generation                                     */
+    double h2d_ms;        /* host->device copy of the task list                         */
+    double kernel_ms;     /* DFS kernel, HIP events on the launch stream                */
+    double total_ms;      /* the whole call                                             */
+} nq_stats;
+
+/* ---- device ------------------------------------------------------------------------ */
+int nq_device_count(void);            /* cudaGetDeviceCount, n_queens_cuda.cu:103-104     */
+const char *nq_last_error(void);
+int nq_set_device(int device);        /* cudaSetDevice, n_queens_cuda.cu:149              */
+/* Compute units, reported clock (kHz) and "name (gcnArch)" of a device. */
+int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t name_len);
+
+/* Fill cfg with defaults for (N, shallowest prefix depth d0): 64-thread workgroups, the
+ * smallest legal stack depth N-1-d0 (LDS per lane decides occupancy, so no frame is wasted),
+ * chunk 1, and a persistent grid of every workgroup that fits on the device.  The reference's
+ * compile-time presets only decide which (N, rows) pairs its CLI accepts; the CLI here keeps
+ * that check (src/main.cpp:16-19). */
+int nq_config_default(int N, int d0, nq_config *cfg);
+
+/* ---- prefix/task generator (host) ------------------------------------------------------
+ * Replaces partial_n_queens + partial_n_queens_for_odd (src/n_queens.cpp:74-113) as called by
+ * cuda_n_queens (src/n_queens_cuda.cu:91-95): every non-attacking placement of the first
+ * `rows` queens with row 0 in the low half of the columns, plus (odd N) the middle column
+ * with row 1 in the low half, in the reference's emission order, as (cur, left, right)
+ * triples (3 uint32 words per task).  With tasks == NULL or cap too small only the count is
+ * returned in *ntasks.  Rows must be in 1..N. */
+int nq_generate(int N, int rows, uint32_t *tasks, uint64_t cap, uint64_t *ntasks);
+
+/* ---- counting --------------------------------------------------------------------------
+ * Replaces the per-GPU region of cuda_n_queens (src/n_queens_cuda.cu:146-179: H2D copy,
+ * kernel n_queens :9-82, D2H of partial_sum) with one device consuming the whole list.
+ * Counts all completions of every task (3 words each, all at the same N; prefixes of any
+ * depth <= N).  *sum = sum of per-task counts (no symmetry fold).  per_task (nullable) gets the
+ * per-task uint64 counts in task order — the reference's partial_sum vector.  nodes (nullable)
+ * selects the node-counting kernel and returns the DFS node count.  stats is nullable. */
+int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg, uint64_t *sum,
+             uint64_t *per_task, uint64_t *nodes, nq_stats *stats);
+
+/* Asynchronous device-pointer form of nq_count, for callers that keep the task list resident
+ * in HBM: d_tasks (3*ntasks uint32), d_result (2 uint64: [0] += sum, [1] += nodes; the caller
+ * zeroes it) and d_per_task (nullable, ntasks uint64) are device pointers on the current
+ * device; stream is a hipStream_t (NULL = default stream).  Enqueues the work and returns;
+ * the results are valid after the stream synchronises.  Calls on one device share the
+ * library's task cursor and are serialised by the library's device lock; each call must be
+ * allowed to finish on its stream before a call on another stream of the same device is
+ * issued. */
+int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_config *cfg,
+                    uint64_t *d_result, uint64_t *d_per_task, int count_nodes, void *stream);
+
+/* The whole reference pipeline: long long cuda_n_queens(int N, int rows)
+ * (src/n_queens_cuda.cu:84-186): generate, count on the device, symmetry fold x2.
+ * *total = the number of N-Queens solutions.  rows is used as given (the CLI applies the
+ * reference's validation, src/main.cpp:10-29). */
+int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NQ_H */

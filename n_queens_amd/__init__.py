@@ -1,0 +1,189 @@
+"""MI355X-native N-Queens solution counter — Python host mirror of the C ABI (include/nq.h).
+
+The product is ``libnq_hip.so`` (HIP kernels for gfx950 + C ABI + C++ drop-in) and the
+``n_queens`` CLI, both built in-tree by ``n_queens_amd.build``.  This module only binds the C
+ABI with ctypes; it never computes counts itself and raises when the library is missing.
+
+Reference interface mirrored (ygch/n_queens, src/n_queens.h:7-19):
+
+* ``cuda_n_queens(N, rows)``            -> ``long long cuda_n_queens(int N, int rows)``
+* ``partial_n_queens(N, rows)``         -> task list of ``partial_n_queens`` (+ ``_for_odd``)
+* ``count(N, tasks)``                   -> the per-GPU region of ``cuda_n_queens``
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnq_hip.so")
+CLI_PATH = os.path.join(_HERE, "n_queens")
+
+NQ_OK = 0
+NQ_ERR_ARG = -1
+NQ_ERR_NO_DEVICE = -2
+NQ_ERR_HIP = -3
+NQ_ERR_DEPTH = -4
+NQ_ERR_NOMEM = -5
+NQ_ERR_BUSY = -6
+NQ_ERR_STATE = -7
+
+# Every symbol include/nq.h declares (checked by tests/test_abi.py).
+EXPORTS = (
+    "nq_device_count", "nq_last_error", "nq_set_device", "nq_device_info", "nq_config_default",
+    "nq_generate", "nq_count", "nq_count_device", "nq_solve",
+)
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class NQError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"nq status {status}: {msg}")
+        self.status = status
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int), ("stack_depth", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
+                ("chunk", ctypes.c_int), ("grid", ctypes.c_int)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("ntasks", ctypes.c_uint64), ("nodes", ctypes.c_uint64), ("gen_ms", ctypes.c_double),
+                ("h2d_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
+@dataclass
+class Config:
+    """Launch configuration (0 = library default).  See nq_config in include/nq.h."""
+    block: int = 0
+    stack_depth: int = 0
+    waves_per_cu: int = 0
+    chunk: int = 0
+    grid: int = 0
+
+    def _c(self) -> _Config:
+        return _Config(self.block, self.stack_depth, self.waves_per_cu, self.chunk, self.grid)
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libnq_hip.so.  Raises if it has not been built (there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m n_queens_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.nq_device_count.restype = ctypes.c_int
+    L.nq_last_error.restype = ctypes.c_char_p
+    L.nq_set_device.argtypes = [ctypes.c_int]
+    L.nq_device_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                 ctypes.c_char_p, ctypes.c_size_t]
+    L.nq_config_default.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Config)]
+    L.nq_generate.argtypes = [ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_uint64, _u64p]
+    L.nq_count.argtypes = [ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.POINTER(_Config), _u64p, _u64p, _u64p,
+                           ctypes.POINTER(_Stats)]
+    L.nq_count_device.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(_Config),
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    L.nq_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Config), _u64p, ctypes.POINTER(_Stats)]
+    for name in EXPORTS:
+        if name not in ("nq_last_error",):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != NQ_OK:
+        raise NQError(rc, lib().nq_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    return int(lib().nq_device_count())
+
+
+def device_info(device: int = 0) -> dict:
+    cus, clk = ctypes.c_int(), ctypes.c_int()
+    name = ctypes.create_string_buffer(256)
+    _check(lib().nq_device_info(device, ctypes.byref(cus), ctypes.byref(clk), name, 256))
+    return {"cus": cus.value, "clock_khz": clk.value, "name": name.value.decode()}
+
+
+def config_default(N: int, d0: int) -> Config:
+    c = _Config()
+    _check(lib().nq_config_default(N, d0, ctypes.byref(c)))
+    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid)
+
+
+def generate(N: int, rows: int) -> np.ndarray:
+    """Task list (ntasks, 3) uint32 of partial_n_queens + partial_n_queens_for_odd."""
+    L = lib()
+    n = ctypes.c_uint64()
+    _check(L.nq_generate(N, rows, None, 0, ctypes.byref(n)))
+    out = np.zeros((max(n.value, 1), 3), dtype=np.uint32)
+    _check(L.nq_generate(N, rows, out.ctypes.data_as(_u32p), n.value, ctypes.byref(n)))
+    return out[: n.value]
+
+
+partial_n_queens = generate
+
+
+@dataclass
+class CountResult:
+    sum: int
+    per_task: np.ndarray | None
+    nodes: int | None
+    kernel_ms: float
+    h2d_ms: float
+    total_ms: float
+
+
+def count(N: int, tasks: np.ndarray, cfg: Config | None = None, per_task: bool = False,
+          nodes: bool = False) -> CountResult:
+    """Count completions of every task on the GPU (nq_count)."""
+    t = np.ascontiguousarray(tasks, dtype=np.uint32).reshape(-1, 3)
+    n = t.shape[0]
+    s = ctypes.c_uint64()
+    nd = ctypes.c_uint64()
+    pt = np.zeros(max(n, 1), dtype=np.uint64) if per_task else None
+    st = _Stats()
+    cc = cfg._c() if cfg else None
+    _check(lib().nq_count(N, t.ctypes.data_as(_u32p), n, ctypes.byref(cc) if cc else None, ctypes.byref(s),
+                          pt.ctypes.data_as(_u64p) if per_task else None, ctypes.byref(nd) if nodes else None,
+                          ctypes.byref(st)))
+    return CountResult(int(s.value), pt[:n] if per_task else None, int(nd.value) if nodes else None,
+                       st.kernel_ms, st.h2d_ms, st.total_ms)
+
+
+def count_device(N: int, d_tasks: int, ntasks: int, d_result: int, d_per_task: int = 0, cfg: Config | None = None,
+                 count_nodes: bool = False, stream: int = 0) -> None:
+    """Enqueue nq_count_device on raw device pointers (ints) and a hipStream_t (int)."""
+    cc = cfg._c() if cfg else None
+    _check(lib().nq_count_device(N, ctypes.c_void_p(d_tasks), ntasks, ctypes.byref(cc) if cc else None,
+                                 ctypes.c_void_p(d_result), ctypes.c_void_p(d_per_task or None),
+                                 1 if count_nodes else 0, ctypes.c_void_p(stream or None)))
+
+
+def solve(N: int, rows: int, cfg: Config | None = None) -> tuple[int, dict]:
+    """Number of N-Queens solutions with `rows` pre-placed (nq_solve = cuda_n_queens)."""
+    tot = ctypes.c_uint64()
+    st = _Stats()
+    cc = cfg._c() if cfg else None
+    _check(lib().nq_solve(N, rows, ctypes.byref(cc) if cc else None, ctypes.byref(tot), ctypes.byref(st)))
+    return int(tot.value), {"ntasks": st.ntasks, "gen_ms": st.gen_ms, "h2d_ms": st.h2d_ms,
+                            "kernel_ms": st.kernel_ms, "total_ms": st.total_ms}
+
+
+def cuda_n_queens(N: int, rows: int) -> int:
+    """Reference-compatible return value: the solution count, or -1 on failure."""
+    try:
+        return solve(N, rows)[0]
+    except NQError:
+        return -1

@@ -1,0 +1,326 @@
+// C ABI of libnq_hip.so (include/nq.h): device management, the counting entry points and the
+// whole reference pipeline.  Replaces the host driver cuda_n_queens
+// (src/n_queens_cuda.cu:84-186) for one device.
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "nq.h"
+#include "nq_internal.h"
+#include "nq_kernels.h"
+
+namespace nq {
+
+namespace {
+thread_local std::string g_err;
+
+struct Device {
+    int id = -1;
+    int cus = 0;
+    int clock_khz = 0;
+    std::string name;
+    DevState *state = nullptr;
+    uint64_t *result = nullptr;  // [2]
+    uint32_t *tasks = nullptr;
+    size_t tasks_cap = 0;  // in tasks
+    uint64_t *per_task = nullptr;
+    size_t per_task_cap = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+};
+
+std::mutex g_devs_mu;
+std::vector<std::unique_ptr<Device>> g_devs;
+
+}  // namespace
+
+int fail(int status, const std::string &msg) {
+    g_err = msg;
+    return status;
+}
+
+int fail_hip(int e, const char *what) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: hipError %d (%s)", what, e, hipGetErrorString(static_cast<hipError_t>(e)));
+    g_err = buf;
+    return NQ_ERR_HIP;
+}
+
+double now_ms() {
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+#define NQ_HIP(call)                                             \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return fail_hip(e_, #call);        \
+    } while (0)
+
+// The calling thread's current device, initialised on first use.
+static int current_device(Device **out) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(NQ_ERR_NO_DEVICE, "Failed to find any gpu!");
+    int id = 0;
+    NQ_HIP(hipGetDevice(&id));
+    std::lock_guard<std::mutex> g(g_devs_mu);
+    if (g_devs.size() < static_cast<size_t>(n)) g_devs.resize(n);
+    if (!g_devs[id]) {
+        auto d = std::make_unique<Device>();
+        d->id = id;
+        hipDeviceProp_t prop;
+        NQ_HIP(hipGetDeviceProperties(&prop, id));
+        d->cus = prop.multiProcessorCount;
+        d->clock_khz = prop.clockRate;
+        d->name = prop.gcnArchName;
+        NQ_HIP(hipMalloc(&d->state, sizeof(DevState)));
+        NQ_HIP(hipMalloc(&d->result, 2 * sizeof(uint64_t)));
+        NQ_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+        NQ_HIP(hipEventCreate(&d->ev0));
+        NQ_HIP(hipEventCreate(&d->ev1));
+        g_devs[id] = std::move(d);
+    }
+    *out = g_devs[id].get();
+    return NQ_OK;
+}
+
+static int check_cfg_shape(const nq_config &c) {
+    if (c.block < 64 || c.block > MAX_BLOCK || c.block % 64) return fail(NQ_ERR_ARG, "block must be 64..256, multiple of 64");
+    if (c.chunk < 1 || c.chunk > 4096) return fail(NQ_ERR_ARG, "chunk must be 1..4096");
+    if (c.stack_depth < 1 || c.stack_depth > 32) return fail(NQ_ERR_ARG, "stack_depth must be 1..32");
+    if (c.waves_per_cu < 0 || c.grid < 0) return fail(NQ_ERR_ARG, "negative waves_per_cu/grid");
+    return NQ_OK;
+}
+
+// Frames a prefix at depth d0 needs: one per depth d0..N-2 that can hold remaining candidates.
+static int frames_needed(int N, int d0) { return d0 + 1 >= N ? 0 : N - 1 - d0; }
+
+// Resolve defaults and the persistent grid for (N, shallowest prefix depth d0, ntasks).
+static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_config *in, bool nodes,
+                          bool per_task, nq_config *out) {
+    nq_config c;
+    nq_config_default(N, d0, &c);
+    if (in) {
+        if (in->block) c.block = in->block;
+        if (in->stack_depth) c.stack_depth = in->stack_depth;
+        if (in->waves_per_cu) c.waves_per_cu = in->waves_per_cu;
+        if (in->chunk) c.chunk = in->chunk;
+        if (in->grid) c.grid = in->grid;
+    }
+    int rc = check_cfg_shape(c);
+    if (rc) return rc;
+    if (c.stack_depth < frames_needed(N, d0)) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "stack depth %d too small: N=%d with prefix depth %d needs %d frames", c.stack_depth,
+                 N, d0, frames_needed(N, d0));
+        return fail(NQ_ERR_DEPTH, buf);
+    }
+    if (c.grid == 0) {
+        int bpc = 0;
+        NQ_HIP(dfs_occupancy(c.block, c.stack_depth, nodes, per_task, &bpc));
+        if (bpc <= 0) return fail(NQ_ERR_ARG, "configuration does not fit on a CU (LDS)");
+        if (c.waves_per_cu > 0) bpc = std::max(1, std::min(bpc, c.waves_per_cu / (c.block / 64)));
+        c.grid = d->cus * bpc;
+    }
+    const uint64_t lanes_needed = (ntasks + c.block - 1) / c.block;
+    if (static_cast<uint64_t>(c.grid) > lanes_needed) c.grid = static_cast<int>(std::max<uint64_t>(1, lanes_needed));
+    *out = c;
+    return NQ_OK;
+}
+
+static int ensure_tasks(Device *d, uint64_t n) {
+    if (d->tasks_cap >= n && d->tasks) return NQ_OK;
+    if (d->tasks) (void)hipFree(d->tasks);
+    d->tasks = nullptr;
+    d->tasks_cap = 0;
+    const size_t cap = std::max<uint64_t>(n, 1);
+    NQ_HIP(hipMalloc(&d->tasks, cap * 3 * sizeof(uint32_t)));
+    d->tasks_cap = cap;
+    return NQ_OK;
+}
+
+static int ensure_per_task(Device *d, uint64_t n) {
+    if (d->per_task_cap >= n && d->per_task) return NQ_OK;
+    if (d->per_task) (void)hipFree(d->per_task);
+    d->per_task = nullptr;
+    d->per_task_cap = 0;
+    const size_t cap = std::max<uint64_t>(n, 1);
+    NQ_HIP(hipMalloc(&d->per_task, cap * sizeof(uint64_t)));
+    d->per_task_cap = cap;
+    return NQ_OK;
+}
+
+static int min_depth(int N, const uint32_t *tasks, uint64_t ntasks) {
+    int d0 = N;
+    for (uint64_t i = 0; i < ntasks; ++i) d0 = std::min(d0, __builtin_popcount(tasks[3 * i]));
+    return d0;
+}
+
+}  // namespace nq
+
+using namespace nq;
+
+extern "C" {
+
+const char *nq_last_error(void) { return g_err.c_str(); }
+
+int nq_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int nq_set_device(int device) {
+    int n = nq_device_count();
+    if (n <= 0) return fail(NQ_ERR_NO_DEVICE, "Failed to find any gpu!");
+    if (device < 0 || device >= n) return fail(NQ_ERR_ARG, "device index out of range");
+    NQ_HIP(hipSetDevice(device));
+    return NQ_OK;
+}
+
+int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t name_len) {
+    int n = nq_device_count();
+    if (n <= 0) return fail(NQ_ERR_NO_DEVICE, "Failed to find any gpu!");
+    if (device < 0 || device >= n) return fail(NQ_ERR_ARG, "device index out of range");
+    hipDeviceProp_t prop;
+    NQ_HIP(hipGetDeviceProperties(&prop, device));
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    if (name && name_len) {
+        snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    return NQ_OK;
+}
+
+int nq_config_default(int N, int d0, nq_config *cfg) {
+    if (!cfg) return fail(NQ_ERR_ARG, "cfg is NULL");
+    cfg->block = 64;
+    cfg->stack_depth = std::max(1, frames_needed(N, d0));
+    cfg->waves_per_cu = 0;
+    cfg->chunk = 1;
+    cfg->grid = 0;
+    return NQ_OK;
+}
+
+int nq_generate(int N, int rows, uint32_t *tasks, uint64_t cap, uint64_t *ntasks) {
+    if (!ntasks) return fail(NQ_ERR_ARG, "ntasks is NULL");
+    if (N < 1 || N > 31) return fail(NQ_ERR_ARG, "N must be 1..31");
+    if (rows < 1 || rows > N) return fail(NQ_ERR_ARG, "rows must be 1..N");
+    std::vector<uint32_t> out;
+    generate_tasks(N, rows, out);
+    const uint64_t n = out.size() / 3;
+    *ntasks = n;
+    if (tasks && cap >= n) memcpy(tasks, out.data(), out.size() * sizeof(uint32_t));
+    return NQ_OK;
+}
+
+int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_config *cfg, uint64_t *d_result,
+                    uint64_t *d_per_task, int count_nodes, void *stream) {
+    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..27 for the GPU kernel");
+    if (ntasks && !d_tasks) return fail(NQ_ERR_ARG, "d_tasks is NULL");
+    if (!d_result) return fail(NQ_ERR_ARG, "d_result is NULL");
+    Device *d = nullptr;
+    int rc = current_device(&d);
+    if (rc) return rc;
+    // The task depths are not visible from the host here: size the stack for the shallowest
+    // prefix the caller allows (cfg->stack_depth), else for depth 1; the kernel flags and skips
+    // any task that would need more.
+    nq_config c;
+    const int d0 = (cfg && cfg->stack_depth) ? std::max(0, N - 1 - cfg->stack_depth) : 1;
+    rc = resolve_config(d, N, d0, ntasks, cfg, count_nodes != 0, d_per_task != nullptr, &c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(d->mu);
+    LaunchArgs a{d_tasks, ntasks, N, c.block, c.grid, c.stack_depth, c.chunk, d->state, d_per_task, d_result,
+                 count_nodes != 0};
+    hipError_t e = launch_count(a, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_hip(e, "launch_count");
+    return NQ_OK;
+}
+
+int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg, uint64_t *sum, uint64_t *per_task,
+             uint64_t *nodes, nq_stats *stats) {
+    const double t0 = now_ms();
+    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..27 for the GPU kernel");
+    if (!sum) return fail(NQ_ERR_ARG, "sum is NULL");
+    if (ntasks && !tasks) return fail(NQ_ERR_ARG, "tasks is NULL");
+    Device *d = nullptr;
+    int rc = current_device(&d);
+    if (rc) return rc;
+    const int d0 = min_depth(N, tasks, ntasks);
+    nq_config c;
+    rc = resolve_config(d, N, d0, ntasks, cfg, nodes != nullptr, per_task != nullptr, &c);
+    if (rc) return rc;
+
+    std::lock_guard<std::mutex> g(d->mu);
+    rc = ensure_tasks(d, ntasks);
+    if (rc) return rc;
+    if (per_task) {
+        rc = ensure_per_task(d, ntasks);
+        if (rc) return rc;
+    }
+    const double t1 = now_ms();
+    if (ntasks)
+        NQ_HIP(hipMemcpyAsync(d->tasks, tasks, ntasks * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream));
+    NQ_HIP(hipMemsetAsync(d->result, 0, 2 * sizeof(uint64_t), d->stream));
+    NQ_HIP(hipStreamSynchronize(d->stream));
+    const double t2 = now_ms();
+    LaunchArgs a{d->tasks, ntasks, N, c.block, c.grid, c.stack_depth, c.chunk, d->state,
+                 per_task ? d->per_task : nullptr, d->result, nodes != nullptr};
+    NQ_HIP(hipEventRecord(d->ev0, d->stream));
+    hipError_t e = launch_count(a, d->stream);
+    if (e != hipSuccess) return fail_hip(e, "launch_count");
+    NQ_HIP(hipEventRecord(d->ev1, d->stream));
+    uint64_t res[2] = {0, 0};
+    DevState st;
+    NQ_HIP(hipMemcpyAsync(res, d->result, sizeof res, hipMemcpyDeviceToHost, d->stream));
+    NQ_HIP(hipMemcpyAsync(&st, d->state, sizeof st, hipMemcpyDeviceToHost, d->stream));
+    if (per_task && ntasks)
+        NQ_HIP(hipMemcpyAsync(per_task, d->per_task, ntasks * sizeof(uint64_t), hipMemcpyDeviceToHost, d->stream));
+    NQ_HIP(hipStreamSynchronize(d->stream));
+    if (st.error) return fail(NQ_ERR_DEPTH, "a task needed more stack frames than configured");
+    *sum = res[0];
+    if (nodes) *nodes = res[1];
+    if (stats) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, d->ev0, d->ev1);
+        stats->ntasks = ntasks;
+        stats->nodes = nodes ? res[1] : 0;
+        stats->gen_ms = 0;
+        stats->h2d_ms = t2 - t1;
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t0;
+    }
+    return NQ_OK;
+}
+
+int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats) {
+    const double t0 = now_ms();
+    if (!total) return fail(NQ_ERR_ARG, "total is NULL");
+    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..27 for the GPU kernel");
+    if (rows < 1 || rows > N) return fail(NQ_ERR_ARG, "rows must be 1..N");
+    std::vector<uint32_t> tasks;
+    generate_tasks(N, rows, tasks);
+    const double t1 = now_ms();
+    const uint64_t n = tasks.size() / 3;
+    uint64_t sum = 0;
+    int rc = nq_count(N, tasks.data(), n, cfg, &sum, nullptr, nullptr, stats);
+    if (rc) return rc;
+    *total = 2 * sum;  // symmetry fold, src/n_queens_cuda.cu:181-183
+    if (stats) {
+        stats->gen_ms = t1 - t0;
+        stats->total_ms = now_ms() - t0;
+    }
+    return NQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// `n_queens N rows [--config 1|2|3]` — drop-in for the reference CLI (src/main.cpp:8-57).
+//
+// Same positional arguments, same validation order and messages (src/main.cpp:10-29), same
+// timed call and result line (src/main.cpp:49-54).  The reference picks a (threads, stack
+// depth) preset at compile time (-D_USE_CONFIGn_, src/macro.h:25-35; compile.sh uses CONFIG2);
+// here --config selects the preset at run time and only its stack depth is used, for the
+// legality check.  Missing arguments print a usage line (the reference reads argv unchecked).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include "n_queens.h"
+
+namespace nq {
+void log_line(const char *fmt, ...);
+}
+
+static float time_diff_ms(const timeval &s, const timeval &e) {
+    return (e.tv_sec - s.tv_sec) * 1000.0 + (e.tv_usec - s.tv_usec) / 1000.0;  // src/utils.cpp:10-12
+}
+
+int main(int argc, char *argv[]) {
+    int config = 2;
+    int pos[2];
+    int npos = 0;
+    for (int i = 1; i < argc; ++i) {
+        if (!strcmp(argv[i], "--config") && i + 1 < argc) {
+            config = atoi(argv[++i]);
+        } else if (npos < 2) {
+            pos[npos++] = atoi(argv[i]);
+        }
+    }
+    if (npos < 2 || config < 1 || config > 3) {
+        fprintf(stderr, "usage: %s N rows [--config 1|2|3]\n", argv[0]);
+        return 2;
+    }
+    static const int kStack[4] = {0, 24, 19, 16};  // STACKSIZE of CONFIG1/2/3, src/macro.h:25-35
+    const int STACKSIZE = kStack[config];
+
+    int N = pos[0];
+    if (N < 2 || N > 28) {
+        printf("Not allowed %d queens problem!\n", N);
+        return -1;
+    }
+    int rows = pos[1];
+    if (STACKSIZE + 1 + rows < N) {
+        printf("This configure cannot process %d-queens with pre-placing %d rows. Pre-placed rows should be >= %d.\n",
+               N, rows, N - STACKSIZE - 1);
+        return -1;
+    }
+    if (N - rows < 2) {
+        printf("Pre-placed rows should be <= %d.\n", N - 2);
+        return -1;
+    }
+    if (rows < 2) {
+        printf("Pre-placed rows shold be at least 2, so set to 2!\n");
+        rows = 2;
+    }
+
+    timeval start, end;
+    nq::log_line("===============================================================\n");
+    gettimeofday(&start, nullptr);
+    long long sum = cuda_n_queens(N, rows);
+    gettimeofday(&end, nullptr);
+    nq::log_line("cuda %d queens result %lld, calc time: [%.2fms]\n", N, sum, time_diff_ms(start, end));
+    nq::log_line("===============================================================\n");
+    return 0;
+}

@@ -1,0 +1,287 @@
+// DFS counting kernel for gfx950 (CDNA4, wave64).
+//
+// Work: every task is a prefix (cur, left, right) of queens already placed in the first rows;
+// the kernel counts all completions of it.  Reference semantics: src/n_queens.cpp:8-21 (what
+// is counted) and src/n_queens_cuda.cu:9-82 (the GPU kernel it replaces).
+//
+// Design (one lane = one independent DFS, 64 lanes per wave):
+//  * The frame being expanded lives in VGPRs: c (columns + depth), l, r (diagonals), v
+//    (candidates not yet tried).  One loop iteration places exactly one queen (one DFS node)
+//    on every active lane; there is no per-iteration reload of the top frame.
+//  * Before a frame is replaced by its child it is pushed to LDS only if it still has
+//    untried candidates (tail-call elimination); a lane whose frame runs empty pops.
+//    Frames are 16 B {c, l, r, v} at stack[level][threadIdx.x]: lanes of a wave touch 64
+//    consecutive 16-B slots, so ds_write_b128 / ds_read_b128 are bank-conflict free whatever
+//    depth each lane is at.
+//  * The depth rides in bits 27..31 of c: placing queen p is c + p + (1 << 27) (p is never in
+//    c), and "the child is at the last row" is the single compare c >= (N-1) << 27.  The last
+//    row is counted by popcount of the child's candidates (0 or 1), as the reference does.
+//    A child with one last-row candidate is expanded once more instead of being cleared
+//    (that child is a solution, 1-2 % of nodes); this keeps the leaf path to compare + bcnt.
+//  * A lane whose task is exhausted claims the next one at once, in the same loop (lane-
+//    granular refill), from a global cursor; claims are aggregated per wave and handed out in
+//    chunks through a wave-private LDS range.
+//  * Per-lane counts are 32-bit (a lane adds at most 1 per iteration) and folded into 64-bit
+//    every 2^30 iterations and at task end; the 64-bit sums are reduced per wave with DPP
+//    shuffles, per workgroup through LDS, and added with one atomic per workgroup.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nq_kernels.h"
+
+namespace nq {
+
+namespace {
+
+constexpr uint32_t DEPTH_ONE = 1u << DEPTH_SHIFT;
+constexpr uint32_t FLUSH_EVERY = 1u << 30;
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ __forceinline__ uint64_t first_lane_u64(uint64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Claim one task index for every active lane of the wave.  wq[0..1] is the wave's claimed but
+// unused range [next, end) in LDS (kept in memory, not registers, because lanes leave the
+// claiming loop at different times and must all see the same range afterwards).
+__device__ __forceinline__ uint64_t claim(unsigned long long *wq, unsigned long long *cursor,
+                                          uint32_t chunk) {
+    const uint64_t mask = __ballot(1);
+    const uint32_t k = __popcll(mask);
+    const uint32_t rank = lane_rank(mask);
+    const uint64_t next = wq[0];
+    const uint64_t end = wq[1];
+    const uint64_t avail = end - next;
+    uint64_t mine;
+    if (avail >= k) {
+        mine = next + rank;
+        if (rank == 0) wq[0] = next + k;
+    } else {
+        const uint64_t want = (k - avail) > chunk ? (k - avail) : chunk;
+        uint64_t g = 0;
+        if (rank == 0) g = atomicAdd(cursor, static_cast<unsigned long long>(want));
+        g = first_lane_u64(g);
+        mine = rank < avail ? next + rank : g + (rank - avail);
+        if (rank == 0) {
+            wq[0] = g + (k - avail);
+            wq[1] = g + want;
+        }
+    }
+    return mine;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+template <bool NODES, bool PER_TASK>
+__global__ void __launch_bounds__(MAX_BLOCK)
+dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint32_t chunk,
+           uint32_t stack_depth, DevState *__restrict__ st, uint64_t *__restrict__ per_task,
+           uint64_t *__restrict__ result) {
+    extern __shared__ uint4 stack[];  // [stack_depth][blockDim.x]
+    __shared__ unsigned long long wq[MAX_WAVES_PER_BLOCK][2];
+    __shared__ unsigned long long red[MAX_WAVES_PER_BLOCK][2];
+
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        wq[wave][0] = 0;
+        wq[wave][1] = 0;
+    }
+    const uint32_t last = (1u << N) - 1u;
+    const uint32_t leaf = (N - 1u) << DEPTH_SHIFT;
+    const uint32_t stride = blockDim.x;
+    uint4 *const base = stack + threadIdx.x;
+    uint4 *sp = base;
+
+    uint4 f = make_uint4(0u, 0u, 0u, 0u);  // the frame being expanded: {c, l, r, v}
+    uint32_t cnt = 0;    // last-row hits since the last fold
+    uint32_t nd = 0;     // nodes since the last fold
+    uint64_t tsum = 0;   // folded count of the current task
+    uint64_t acc = 0;    // finished tasks
+    uint64_t nodes = 0;
+    uint64_t tid = ~0ull;
+    uint32_t guard = 0;
+
+    for (;;) {
+        if (f.w == 0) {
+            if (sp == base) {
+                // Current task exhausted (or first entry): account for it, claim the next.
+                const uint64_t done = tsum + cnt;
+                acc += done;
+                if (PER_TASK && tid != ~0ull) per_task[tid] = done;
+                tsum = 0;
+                cnt = 0;
+                bool got = false;
+                for (;;) {
+                    const uint64_t t = claim(wq[wave], &st->cursor, chunk);
+                    if (t >= ntasks) break;
+                    const uint32_t c0 = tasks[3 * t + 0];
+                    const uint32_t l0 = tasks[3 * t + 1];
+                    const uint32_t r0 = tasks[3 * t + 2];
+                    const uint32_t v0 = last & ~(c0 | l0 | r0);
+                    const uint32_t d0 = __popc(c0);
+                    if (d0 + 1u >= N || v0 == 0) {
+                        // Prefix already at the last row (or beyond), or no legal next queen:
+                        // its count is immediate (src/n_queens.cpp:10-13 counts a full board).
+                        const uint64_t direct =
+                            d0 >= N ? (c0 == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
+                        acc += direct;
+                        if (PER_TASK) per_task[t] = direct;
+                        continue;
+                    }
+                    if (N - 1u - d0 > stack_depth) {
+                        // Not enough frames for this prefix: flag and skip (host checks first).
+                        atomicOr(&st->error, 1ull);
+                        if (PER_TASK) per_task[t] = 0;
+                        continue;
+                    }
+                    f = make_uint4(c0 | (d0 << DEPTH_SHIFT), l0, r0, v0);
+                    tid = t;
+                    got = true;
+                    break;
+                }
+                if (!got) break;  // no tasks left: this lane retires
+            } else {
+                sp -= stride;
+                f = *sp;
+            }
+        }
+        // Expand one node: place the lowest candidate p; keep the frame (with p removed) on the
+        // stack only if candidates remain, then turn the registers into the child.
+        const uint32_t p = f.w & (0u - f.w);
+        if (NODES) nd += (f.x < leaf) ? 1u : 0u;
+        f.w ^= p;
+        if (f.w != 0) {
+            *sp = f;
+            sp += stride;
+        }
+        f.x = f.x + p + DEPTH_ONE;
+        f.y = (f.y | p) << 1;
+        f.z = (f.z | p) >> 1;
+        f.w = last & ~(f.x | f.y | f.z);
+        // Child at the last row: its candidates (0 or 1) are the completions found here.
+        const uint32_t hit = f.x >= leaf ? f.w : 0u;
+        __asm__("v_bcnt_u32_b32 %0, %1, %2" : "=v"(cnt) : "v"(hit), "v"(cnt));
+        if (__builtin_expect(++guard == FLUSH_EVERY, 0)) {
+            // Rare, wave-uniform fold; the empty volatile asm keeps it a real branch.
+            __asm__ volatile("" ::: "memory");
+            guard = 0;
+            tsum += cnt;
+            cnt = 0;
+            if (NODES) {
+                nodes += nd;
+                nd = 0;
+            }
+        }
+    }
+    if (NODES) nodes += nd;
+
+    // Reduce: wave (shuffles), workgroup (LDS), device (one atomic per workgroup).
+    acc = wave_sum_u64(acc);
+    if (NODES) nodes = wave_sum_u64(nodes);
+    if ((threadIdx.x & 63u) == 0) {
+        red[wave][0] = acc;
+        red[wave][1] = nodes;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0, n = 0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
+            s += red[w][0];
+            n += red[w][1];
+        }
+        atomicAdd(&st->total, s);
+        if (NODES) atomicAdd(&st->nodes, n);
+    }
+}
+
+__global__ void reset_state_kernel(DevState *st) {
+    if (threadIdx.x == 0) {
+        st->cursor = 0;
+        st->total = 0;
+        st->nodes = 0;
+        st->error = 0;
+    }
+}
+
+__global__ void fold_result_kernel(const DevState *st, uint64_t *result) {
+    if (threadIdx.x == 0) {
+        result[0] += st->total;
+        result[1] += st->nodes;
+    }
+}
+
+template <bool NODES, bool PER_TASK>
+void *kernel_ptr() {
+    return reinterpret_cast<void *>(&dfs_kernel<NODES, PER_TASK>);
+}
+
+const void *select_kernel(bool nodes, bool per_task) {
+    return nodes ? (per_task ? kernel_ptr<true, true>() : kernel_ptr<true, false>())
+                 : (per_task ? kernel_ptr<false, true>() : kernel_ptr<false, false>());
+}
+
+// Dynamic LDS above 64 KiB must be allowed per kernel (gfx950 has 160 KiB per CU).
+hipError_t allow_lds(const void *k, size_t lds) {
+    if (lds <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+}
+
+}  // namespace
+
+size_t dfs_lds_bytes(int block, int stack_depth) {
+    return static_cast<size_t>(block) * static_cast<size_t>(stack_depth) * sizeof(uint4);
+}
+
+hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task,
+                         int *blocks_per_cu) {
+    const void *k = select_kernel(count_nodes, per_task);
+    const size_t lds = dfs_lds_bytes(block, stack_depth);
+    hipError_t e = allow_lds(k, lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, block, lds);
+}
+
+hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
+    reset_state_kernel<<<1, 64, 0, stream>>>(a.state);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t lds = dfs_lds_bytes(a.block, a.stack_depth);
+    e = allow_lds(select_kernel(a.count_nodes, a.per_task != nullptr), lds);
+    if (e != hipSuccess) return e;
+    const uint32_t N = static_cast<uint32_t>(a.N);
+    const uint32_t chunk = static_cast<uint32_t>(a.chunk);
+    const uint32_t sd = static_cast<uint32_t>(a.stack_depth);
+    if (a.count_nodes) {
+        if (a.per_task)
+            dfs_kernel<true, true><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
+                                                                      a.per_task, a.result);
+        else
+            dfs_kernel<true, false><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
+                                                                       a.per_task, a.result);
+    } else {
+        if (a.per_task)
+            dfs_kernel<false, true><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
+                                                                       a.per_task, a.result);
+        else
+            dfs_kernel<false, false><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
+                                                                        a.per_task, a.result);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (a.result) {
+        fold_result_kernel<<<1, 64, 0, stream>>>(a.state, a.result);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+}  // namespace nq

@@ -1,0 +1,114 @@
+"""Host-side product checks that need no GPU: the C ABI library loads and exports every symbol
+include/nq.h declares, the product generator equals the oracle/reference task lists, the C++
+drop-in links, and the CLI reproduces the reference's argument validation."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import n_queens_amd as nq
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header: str) -> set[str]:
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(nq_\w+)\s*\(", text, flags=re.M))
+
+
+def test_library_exports_every_declared_symbol():
+    L = nq.lib()
+    decl = declared_functions("nq.h")
+    assert decl, "no declarations parsed"
+    assert decl == set(nq.EXPORTS), decl ^ set(nq.EXPORTS)
+    for name in decl:
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", nq.LIB_PATH], capture_output=True, text=True).stdout
+    for cxx in ("_Z13cuda_n_queensii", "_Z16partial_n_queensiiiiRSt6vectorIiSaIiEEi",
+                "_Z24partial_n_queens_for_oddiiiiRSt6vectorIiSaIiEEi"):
+        assert cxx in out, cxx
+
+
+@pytest.mark.parametrize("N", range(2, 18))
+def test_product_generator_equals_oracle(N):
+    for rows in range(1, min(N, 7) + 1):
+        assert np.array_equal(nq.generate(N, rows), oracle.gen_tasks(N, rows)), (N, rows)
+
+
+def test_product_generator_vs_reference_vectors(ref_vectors):
+    for N, rows, _ in ref_vectors["totals"]:
+        N, rows = int(N), int(rows)
+        assert np.array_equal(nq.generate(N, rows), ref_vectors[f"N{N}_r{rows}_tasks"])
+
+
+def test_product_generator_task_counts_vs_logs(published):
+    for N, rows, k, _src in published["task_counts"]:
+        if N <= 22:
+            assert len(nq.generate(N, rows)) == k, (N, rows)
+
+
+def test_product_generator_large_matches_oracle():
+    a = nq.generate(21, 6)
+    assert len(a) == 4470757
+    assert np.array_equal(a, oracle.gen_tasks(21, 6))
+
+
+def test_generate_rejects_bad_args():
+    with pytest.raises(nq.NQError):
+        nq.generate(0, 1)
+    with pytest.raises(nq.NQError):
+        nq.generate(8, 9)
+
+
+def test_cpp_dropin_links_and_generates(tmp_path):
+    """A reference-style caller links against libnq_hip.so through include/n_queens.h."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        '#include "n_queens.h"\n#include <cstdio>\n'
+        "int main(){ vector<int> t; partial_n_queens(9,0,0,0,t,3); partial_n_queens_for_odd(9,0,0,0,t,3);\n"
+        ' printf("%zu", t.size()/3); for (int x : t) printf(" %d", x); printf("\\n"); return 0; }\n')
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(nq.LIB_PATH)
+    subprocess.check_call(["g++", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                           "-L", libdir, "-lnq_hip", f"-Wl,-rpath,{libdir}"])
+    words = [int(x) for x in subprocess.check_output([str(exe)]).split()]
+    expect = oracle.gen_tasks(9, 3)
+    assert words[0] == len(expect)
+    assert np.array_equal(np.array(words[1:], dtype=np.int64).astype(np.uint32).reshape(-1, 3), expect)
+
+
+NO_GPU = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+
+
+def run_cli(*args, env=NO_GPU):
+    # The device is hidden so that no case here starts a real count, also on a GPU machine.
+    return subprocess.run([nq.CLI_PATH, *map(str, args)], capture_output=True, text=True, env=env, timeout=60)
+
+
+def test_cli_validation_order():
+    # src/main.cpp:10-29, in order.
+    p = run_cli(29, 5)
+    assert p.returncode == 255 and p.stdout.strip() == "Not allowed 29 queens problem!"
+    p = run_cli(1, 0)
+    assert p.stdout.strip() == "Not allowed 1 queens problem!"
+    p = run_cli(25, 3)  # CONFIG2: STACKSIZE 19, 19+1+3 < 25
+    assert p.stdout.strip() == ("This configure cannot process 25-queens with pre-placing 3 rows. "
+                                "Pre-placed rows should be >= 5.")
+    p = run_cli(25, 3, "--config", 1)  # CONFIG1: STACKSIZE 24 accepts it; rows fine -> would run
+    assert "This configure" not in p.stdout
+    p = run_cli(13, 12)
+    assert p.stdout.strip() == "Pre-placed rows should be <= 11."
+
+
+def test_cli_no_gpu_path():
+    p = run_cli(13, 1)
+    lines = p.stdout.splitlines()
+    assert lines[0] == "Pre-placed rows shold be at least 2, so set to 2!"
+    assert re.match(r"\[\d{4}-\d\d-\d\d \d\d:\d\d:\d\d\.\d{3}\] =+$", lines[1])
+    assert re.search(r"Use [\d.]+ms to generate 66 subproblems!", lines[2])
+    assert "Failed to find any gpu!" in p.stdout
+    assert re.search(r"cuda 13 queens result -1, calc time: \[[\d.]+ms\]", p.stdout)
