@@ -51,6 +51,9 @@ typedef struct nq_stats {
     double h2d_ms;        /* host->device copy of the task list                         */
     double kernel_ms;     /* DFS kernel, HIP events on the launch stream                */
     double total_ms;      /* the whole call                                             */
+    double busy_frac;     /* sum of per-wave residency / (waves x kernel span)          */
+    double tail_ms;       /* first wave finished -> last wave finished                  */
+    uint64_t waves;       /* resident waves of the persistent grid                      */
 } nq_stats;
 
 /* ---- device ------------------------------------------------------------------------ */

@@ -54,7 +54,8 @@ class _Config(ctypes.Structure):
 
 class _Stats(ctypes.Structure):
     _fields_ = [("ntasks", ctypes.c_uint64), ("nodes", ctypes.c_uint64), ("gen_ms", ctypes.c_double),
-                ("h2d_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+                ("h2d_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("busy_frac", ctypes.c_double), ("tail_ms", ctypes.c_double), ("waves", ctypes.c_uint64)]
 
 
 @dataclass
@@ -143,6 +144,9 @@ class CountResult:
     kernel_ms: float
     h2d_ms: float
     total_ms: float
+    busy_frac: float = 0.0
+    tail_ms: float = 0.0
+    waves: int = 0
 
 
 def count(N: int, tasks: np.ndarray, cfg: Config | None = None, per_task: bool = False,
@@ -159,7 +163,7 @@ def count(N: int, tasks: np.ndarray, cfg: Config | None = None, per_task: bool =
                           pt.ctypes.data_as(_u64p) if per_task else None, ctypes.byref(nd) if nodes else None,
                           ctypes.byref(st)))
     return CountResult(int(s.value), pt[:n] if per_task else None, int(nd.value) if nodes else None,
-                       st.kernel_ms, st.h2d_ms, st.total_ms)
+                       st.kernel_ms, st.h2d_ms, st.total_ms, st.busy_frac, st.tail_ms, int(st.waves))
 
 
 def count_device(N: int, d_tasks: int, ntasks: int, d_result: int, d_per_task: int = 0, cfg: Config | None = None,
@@ -178,7 +182,8 @@ def solve(N: int, rows: int, cfg: Config | None = None) -> tuple[int, dict]:
     cc = cfg._c() if cfg else None
     _check(lib().nq_solve(N, rows, ctypes.byref(cc) if cc else None, ctypes.byref(tot), ctypes.byref(st)))
     return int(tot.value), {"ntasks": st.ntasks, "gen_ms": st.gen_ms, "h2d_ms": st.h2d_ms,
-                            "kernel_ms": st.kernel_ms, "total_ms": st.total_ms}
+                            "kernel_ms": st.kernel_ms, "total_ms": st.total_ms, "busy_frac": st.busy_frac,
+                            "tail_ms": st.tail_ms, "waves": int(st.waves)}
 
 
 def cuda_n_queens(N: int, rows: int) -> int:

@@ -35,6 +35,9 @@ struct Device {
     size_t per_task_cap = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int wall_khz = 0;  // rate of the kernel's wall_clock64 counter
+    uint64_t *trace = nullptr;
+    size_t trace_cap = 0;  // in uint64
     std::mutex mu;
 };
 
@@ -88,6 +91,7 @@ static int current_device(Device **out) {
         NQ_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
         NQ_HIP(hipEventCreate(&d->ev0));
         NQ_HIP(hipEventCreate(&d->ev1));
+        NQ_HIP(hipDeviceGetAttribute(&d->wall_khz, hipDeviceAttributeWallClockRate, id));
         g_devs[id] = std::move(d);
     }
     *out = g_devs[id].get();
@@ -158,6 +162,37 @@ static int ensure_per_task(Device *d, uint64_t n) {
     NQ_HIP(hipMalloc(&d->per_task, cap * sizeof(uint64_t)));
     d->per_task_cap = cap;
     return NQ_OK;
+}
+
+static int ensure_trace(Device *d, size_t n) {
+    if (d->trace_cap >= n && d->trace) return NQ_OK;
+    if (d->trace) (void)hipFree(d->trace);
+    d->trace = nullptr;
+    d->trace_cap = 0;
+    NQ_HIP(hipMalloc(&d->trace, n * sizeof(uint64_t)));
+    d->trace_cap = n;
+    return NQ_OK;
+}
+
+// Per-wave {start, end} stamps -> fraction of wave-time busy and the drain tail (ms).
+static void trace_stats(const std::vector<uint64_t> &tr, int wall_khz, double *busy, double *tail_ms) {
+    const size_t w = tr.size() / 2;
+    if (!w || wall_khz <= 0) {
+        *busy = 0;
+        *tail_ms = 0;
+        return;
+    }
+    uint64_t t0 = UINT64_MAX, t1 = 0, first_end = UINT64_MAX;
+    double sum = 0;
+    for (size_t i = 0; i < w; ++i) {
+        t0 = std::min(t0, tr[2 * i]);
+        t1 = std::max(t1, tr[2 * i + 1]);
+        first_end = std::min(first_end, tr[2 * i + 1]);
+        sum += static_cast<double>(tr[2 * i + 1] - tr[2 * i]);
+    }
+    const double span = static_cast<double>(t1 - t0);
+    *busy = span > 0 ? sum / (span * static_cast<double>(w)) : 1.0;
+    *tail_ms = static_cast<double>(t1 - first_end) / wall_khz;
 }
 
 static int min_depth(int N, const uint32_t *tasks, uint64_t ntasks) {
@@ -268,6 +303,11 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
         rc = ensure_per_task(d, ntasks);
         if (rc) return rc;
     }
+    const size_t nwaves = static_cast<size_t>(c.grid) * (c.block / 64);
+    if (stats) {
+        rc = ensure_trace(d, 2 * nwaves);
+        if (rc) return rc;
+    }
     const double t1 = now_ms();
     if (ntasks)
         NQ_HIP(hipMemcpyAsync(d->tasks, tasks, ntasks * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream));
@@ -275,7 +315,7 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     NQ_HIP(hipStreamSynchronize(d->stream));
     const double t2 = now_ms();
     LaunchArgs a{d->tasks, ntasks, N, c.block, c.grid, c.stack_depth, c.chunk, d->state,
-                 per_task ? d->per_task : nullptr, d->result, nodes != nullptr};
+                 per_task ? d->per_task : nullptr, d->result, nodes != nullptr, stats ? d->trace : nullptr};
     NQ_HIP(hipEventRecord(d->ev0, d->stream));
     hipError_t e = launch_count(a, d->stream);
     if (e != hipSuccess) return fail_hip(e, "launch_count");
@@ -286,6 +326,11 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     NQ_HIP(hipMemcpyAsync(&st, d->state, sizeof st, hipMemcpyDeviceToHost, d->stream));
     if (per_task && ntasks)
         NQ_HIP(hipMemcpyAsync(per_task, d->per_task, ntasks * sizeof(uint64_t), hipMemcpyDeviceToHost, d->stream));
+    std::vector<uint64_t> tr;
+    if (stats) {
+        tr.resize(2 * nwaves);
+        NQ_HIP(hipMemcpyAsync(tr.data(), d->trace, tr.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, d->stream));
+    }
     NQ_HIP(hipStreamSynchronize(d->stream));
     if (st.error) return fail(NQ_ERR_DEPTH, "a task needed more stack frames than configured");
     *sum = res[0];
@@ -299,6 +344,8 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
         stats->h2d_ms = t2 - t1;
         stats->kernel_ms = ms;
         stats->total_ms = now_ms() - t0;
+        trace_stats(tr, d->wall_khz, &stats->busy_frac, &stats->tail_ms);
+        stats->waves = static_cast<uint64_t>(nwaves);
     }
     return NQ_OK;
 }

@@ -34,7 +34,7 @@ namespace nq {
 namespace {
 
 constexpr uint32_t DEPTH_ONE = 1u << DEPTH_SHIFT;
-constexpr uint32_t FLUSH_EVERY = 1u << 30;
+constexpr uint32_t FLUSH_EVERY = 1u << 30;  // hot-loop trips (2 iterations each) between folds
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
@@ -81,109 +81,168 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
     return x;
 }
 
+// One DFS iteration for every live lane of the wave, hand-scheduled (gfx950).  The frame
+// {c, l, r, v} lives in v[20:23] (bound by the asm constraints below) so that it is both the
+// ds_write_b128 source and the ds_read_b128 destination without register moves.
+//
+//   p = v & -v; v ^= p                          lowest candidate, v = candidates left
+//   if (v != 0) { stack[sp] = {c,l,r,v}; sp += stride }      (exec-masked push)
+//   c += p + (1 << 27); l = (l|p) << 1; r = (r|p) >> 1
+//   v = last & ~(l|r) & ~c                     (v_bitop3 table 0x04: ~s0 & s1 & ~s2)
+//   if (c >= leaf) cnt += popcount(v)          (exec-masked)
+//   if (v == 0) { if (sp != base) { sp -= stride; {c,l,r,v} = stack[sp] } else done |= lane }
+//
+// Only SALU instructions read the masks the VALU compares write, and every LDS read is waited
+// for before the iteration ends, so no hazard wait states are needed.  Lanes that become done
+// are removed from exec for the rest of the asm statement; [nd] counts expansions (node-
+// counting variant only).
+#define NQ_ITER(NODES_INSN)                                                                   \
+    "v_sub_u32 %[t], 0, v23\n\t"                                                              \
+    "v_and_b32 %[p], v23, %[t]\n\t"                                                           \
+    "v_xor_b32 v23, v23, %[p]\n\t" NODES_INSN                                                  \
+    "v_cmp_ne_u32 %[m1], 0, v23\n\t"                                                          \
+    "s_and_saveexec_b64 %[tmp], %[m1]\n\t"                                                    \
+    "ds_write_b128 %[sp], v[20:23]\n\t"                                                       \
+    "v_add_u32 %[sp], %[stride], %[sp]\n\t"                                                   \
+    "s_mov_b64 exec, %[tmp]\n\t"                                                              \
+    "v_add3_u32 v20, v20, %[p], %[one]\n\t"                                                   \
+    "v_or_b32 v21, v21, %[p]\n\t"                                                             \
+    "v_lshlrev_b32 v21, 1, v21\n\t"                                                           \
+    "v_or_b32 v22, v22, %[p]\n\t"                                                             \
+    "v_lshrrev_b32 v22, 1, v22\n\t"                                                           \
+    "v_or_b32 %[t], v21, v22\n\t"                                                             \
+    "v_bitop3_b32 v23, %[t], %[last], v20 bitop3:4\n\t"                                       \
+    "v_cmp_le_u32 %[m1], %[leaf], v20\n\t"                                                    \
+    "s_and_saveexec_b64 %[tmp], %[m1]\n\t"                                                    \
+    "v_bcnt_u32_b32 %[cnt], v23, %[cnt]\n\t"                                                  \
+    "s_mov_b64 exec, %[tmp]\n\t"                                                              \
+    "v_cmp_eq_u32 %[m1], 0, v23\n\t"                                                          \
+    "s_and_saveexec_b64 %[tmp], %[m1]\n\t"                                                    \
+    "v_cmp_ne_u32 %[m2], %[sp], %[base]\n\t"                                                  \
+    "s_and_b64 exec, exec, %[m2]\n\t"                                                         \
+    "v_subrev_u32 %[sp], %[stride], %[sp]\n\t"                                                \
+    "ds_read_b128 v[20:23], %[sp]\n\t"                                                        \
+    "s_andn2_b64 %[m1], %[m1], %[m2]\n\t"                                                     \
+    "s_or_b64 %[done], %[done], %[m1]\n\t"                                                    \
+    "s_andn2_b64 exec, %[tmp], %[m1]\n\t"                                                     \
+    "s_waitcnt lgkmcnt(0)\n\t"
+
+// The whole hot loop: two iterations per trip, leaving when some lane becomes done or after
+// [iters] trips (the caller folds the 32-bit counters then).
+#define NQ_HOT_ASM(NODES_INSN)                                                                \
+    __asm__ volatile(                                                                         \
+        "s_mov_b64 %[save], exec\n\t"                                                         \
+        "s_mov_b64 exec, %[live]\n\t"                                                         \
+        "s_mov_b64 %[done], 0\n"                                                              \
+        "nq_hot_%=:\n\t" NQ_ITER(NODES_INSN) NQ_ITER(NODES_INSN)                              \
+        "s_cmp_lg_u64 %[done], 0\n\t"                                                         \
+        "s_cbranch_scc1 nq_exit_%=\n\t"                                                       \
+        "s_sub_u32 %[iters], %[iters], 1\n\t"                                                 \
+        "s_cmp_lg_u32 %[iters], 0\n\t"                                                        \
+        "s_cbranch_scc1 nq_hot_%=\n"                                                          \
+        "nq_exit_%=:\n\t"                                                                     \
+        "s_mov_b64 exec, %[save]\n\t"                                                         \
+        : "+{v20}"(c), "+{v21}"(l), "+{v22}"(r), "+{v23}"(v), [sp] "+v"(sp), [cnt] "+v"(cnt),   \
+          [nd] "+v"(nd), [iters] "+s"(iters), [t] "=&v"(t), [p] "=&v"(p), [m1] "=&s"(m1),       \
+          [m2] "=&s"(m2), [tmp] "=&s"(tmp), [save] "=&s"(save), [done] "=&s"(done)             \
+        : [live] "s"(live), [base] "v"(base), [stride] "s"(stride), [one] "s"(DEPTH_ONE),     \
+          [last] "s"(last), [leaf] "s"(leaf)                                                  \
+        : "memory", "scc")
+
 template <bool NODES, bool PER_TASK>
 __global__ void __launch_bounds__(MAX_BLOCK)
 dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint32_t chunk,
            uint32_t stack_depth, DevState *__restrict__ st, uint64_t *__restrict__ per_task,
-           uint64_t *__restrict__ result) {
+           uint64_t *__restrict__ trace) {
     extern __shared__ uint4 stack[];  // [stack_depth][blockDim.x]
     __shared__ unsigned long long wq[MAX_WAVES_PER_BLOCK][2];
     __shared__ unsigned long long red[MAX_WAVES_PER_BLOCK][2];
 
     const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t t_start = trace ? wall_clock64() : 0;
     if ((threadIdx.x & 63u) == 0) {
         wq[wave][0] = 0;
         wq[wave][1] = 0;
     }
     const uint32_t last = (1u << N) - 1u;
     const uint32_t leaf = (N - 1u) << DEPTH_SHIFT;
-    const uint32_t stride = blockDim.x;
-    uint4 *const base = stack + threadIdx.x;
-    uint4 *sp = base;
+    const uint32_t stride = blockDim.x * static_cast<uint32_t>(sizeof(uint4));
+    const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(stack + threadIdx.x));
+    uint32_t sp = base;
 
-    uint4 f = make_uint4(0u, 0u, 0u, 0u);  // the frame being expanded: {c, l, r, v}
-    uint32_t cnt = 0;    // last-row hits since the last fold
-    uint32_t nd = 0;     // nodes since the last fold
-    uint64_t tsum = 0;   // folded count of the current task
-    uint64_t acc = 0;    // finished tasks
-    uint64_t nodes = 0;
+    uint32_t c = 0, l = 0, r = 0, v = 0;  // the frame being expanded
+    uint32_t cnt = 0;     // last-row hits since the last fold (at most one per iteration)
+    uint32_t nd = 0;      // expansions since the last fold (node-counting variant)
+    uint64_t tsum = 0;    // folded count of the current task
+    uint64_t acc = 0;     // counts of finished tasks
+    uint64_t nodes = 0;   // expansions minus last-row hits, i.e. DFS nodes
     uint64_t tid = ~0ull;
-    uint32_t guard = 0;
+    bool has = true;      // lane still has (or may claim) work
 
     for (;;) {
-        if (f.w == 0) {
-            if (sp == base) {
-                // Current task exhausted (or first entry): account for it, claim the next.
-                const uint64_t done = tsum + cnt;
-                acc += done;
-                if (PER_TASK && tid != ~0ull) per_task[tid] = done;
-                tsum = 0;
-                cnt = 0;
-                bool got = false;
-                for (;;) {
-                    const uint64_t t = claim(wq[wave], &st->cursor, chunk);
-                    if (t >= ntasks) break;
-                    const uint32_t c0 = tasks[3 * t + 0];
-                    const uint32_t l0 = tasks[3 * t + 1];
-                    const uint32_t r0 = tasks[3 * t + 2];
-                    const uint32_t v0 = last & ~(c0 | l0 | r0);
-                    const uint32_t d0 = __popc(c0);
-                    if (d0 + 1u >= N || v0 == 0) {
-                        // Prefix already at the last row (or beyond), or no legal next queen:
-                        // its count is immediate (src/n_queens.cpp:10-13 counts a full board).
-                        const uint64_t direct =
-                            d0 >= N ? (c0 == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
-                        acc += direct;
-                        if (PER_TASK) per_task[t] = direct;
-                        continue;
-                    }
-                    if (N - 1u - d0 > stack_depth) {
-                        // Not enough frames for this prefix: flag and skip (host checks first).
-                        atomicOr(&st->error, 1ull);
-                        if (PER_TASK) per_task[t] = 0;
-                        continue;
-                    }
-                    f = make_uint4(c0 | (d0 << DEPTH_SHIFT), l0, r0, v0);
-                    tid = t;
-                    got = true;
-                    break;
+        // Lanes whose task is exhausted (v == 0 at the stack bottom) claim the next one.
+        if (has && v == 0) {
+            const uint64_t done = tsum;
+            acc += done;
+            if (PER_TASK && tid != ~0ull) per_task[tid] = done;
+            tsum = 0;
+            has = false;
+            for (;;) {
+                const uint64_t t = claim(wq[wave], &st->cursor, chunk);
+                if (t >= ntasks) break;
+                const uint32_t c0 = tasks[3 * t + 0];
+                const uint32_t l0 = tasks[3 * t + 1];
+                const uint32_t r0 = tasks[3 * t + 2];
+                const uint32_t v0 = last & ~(c0 | l0 | r0);
+                const uint32_t d0 = __popc(c0);
+                if (d0 + 1u >= N || v0 == 0) {
+                    // Prefix already at the last row (or beyond), or no legal next queen:
+                    // its count is immediate (src/n_queens.cpp:10-13 counts a full board).
+                    const uint64_t direct =
+                        d0 >= N ? (c0 == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
+                    acc += direct;
+                    if (PER_TASK) per_task[t] = direct;
+                    continue;
                 }
-                if (!got) break;  // no tasks left: this lane retires
-            } else {
-                sp -= stride;
-                f = *sp;
+                if (N - 1u - d0 > stack_depth) {
+                    // Not enough frames for this prefix: flag and skip (host checks first).
+                    atomicOr(&st->error, 1ull);
+                    if (PER_TASK) per_task[t] = 0;
+                    continue;
+                }
+                c = c0 | (d0 << DEPTH_SHIFT);
+                l = l0;
+                r = r0;
+                v = v0;
+                tid = t;
+                has = true;
+                break;
             }
         }
-        // Expand one node: place the lowest candidate p; keep the frame (with p removed) on the
-        // stack only if candidates remain, then turn the registers into the child.
-        const uint32_t p = f.w & (0u - f.w);
-        if (NODES) nd += (f.x < leaf) ? 1u : 0u;
-        f.w ^= p;
-        if (f.w != 0) {
-            *sp = f;
-            sp += stride;
+        const uint64_t live = __ballot(has);
+        if (live == 0) break;
+        // Hot loop until a lane needs a new task; the trip bound keeps the 32-bit per-lane
+        // counters from wrapping (at most 2 * FLUSH_EVERY hits between folds).
+        {
+            uint32_t t, p, iters = FLUSH_EVERY;
+            uint64_t m1, m2, tmp, save, done;
+            if (NODES)
+                NQ_HOT_ASM("v_add_u32 %[nd], 1, %[nd]\n\t");
+            else
+                NQ_HOT_ASM("");
         }
-        f.x = f.x + p + DEPTH_ONE;
-        f.y = (f.y | p) << 1;
-        f.z = (f.z | p) >> 1;
-        f.w = last & ~(f.x | f.y | f.z);
-        // Child at the last row: its candidates (0 or 1) are the completions found here.
-        const uint32_t hit = f.x >= leaf ? f.w : 0u;
-        __asm__("v_bcnt_u32_b32 %0, %1, %2" : "=v"(cnt) : "v"(hit), "v"(cnt));
-        if (__builtin_expect(++guard == FLUSH_EVERY, 0)) {
-            // Rare, wave-uniform fold; the empty volatile asm keeps it a real branch.
-            __asm__ volatile("" ::: "memory");
-            guard = 0;
-            tsum += cnt;
-            cnt = 0;
-            if (NODES) {
-                nodes += nd;
-                nd = 0;
-            }
-        }
+        tsum += cnt;
+        if (NODES) nodes += static_cast<uint64_t>(nd) - cnt;
+        cnt = 0;
+        nd = 0;
     }
-    if (NODES) nodes += nd;
 
+    if (trace && (threadIdx.x & 63u) == 0) {
+        // Per-wave residency (constant-rate wall clock) for the tail-idle statistic.
+        const uint64_t w = static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + wave;
+        trace[2 * w] = t_start;
+        trace[2 * w + 1] = wall_clock64();
+    }
     // Reduce: wave (shuffles), workgroup (LDS), device (one atomic per workgroup).
     acc = wave_sum_u64(acc);
     if (NODES) nodes = wave_sum_u64(nodes);
@@ -263,17 +322,17 @@ hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
     if (a.count_nodes) {
         if (a.per_task)
             dfs_kernel<true, true><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                      a.per_task, a.result);
+                                                                      a.per_task, a.trace);
         else
             dfs_kernel<true, false><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                       a.per_task, a.result);
+                                                                       a.per_task, a.trace);
     } else {
         if (a.per_task)
             dfs_kernel<false, true><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                       a.per_task, a.result);
+                                                                       a.per_task, a.trace);
         else
             dfs_kernel<false, false><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                        a.per_task, a.result);
+                                                                        a.per_task, a.trace);
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;

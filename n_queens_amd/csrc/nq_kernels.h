@@ -34,6 +34,7 @@ struct LaunchArgs {
     uint64_t *per_task;   // nullable
     uint64_t *result;     // nullable: [0] += total, [1] += nodes at the end of the launch
     bool count_nodes;
+    uint64_t *trace = nullptr;  // nullable: per wave {start, end} wall-clock ticks (grid * block/64 * 2)
 };
 
 // Enqueue: reset state, DFS, (optional) fold into result.  Returns a hipError_t.
