@@ -63,6 +63,40 @@ def cpu_baseline(nodes_fn) -> dict | None:
             "seconds": dt, "solutions": total}
 
 
+def batched_mix(nq, log) -> dict:
+    """512 seeded instances (N in [14,20], 2-6 fixed rows): one batched launch vs one plain
+    launch per instance, both host-in/host-out, counts required identical."""
+    from n_queens_amd import workloads
+    mix = workloads.batch_mix()
+    n = len(mix)
+    nq.batch_count(mix[:8])  # warm the batch path
+    nq.instance_count(*mix[0])
+    t0 = time.perf_counter()
+    per = [nq.instance_count(N, cols) for N, cols in mix]
+    t_per = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    counts, st = nq.batch_count(mix)
+    t_batch = time.perf_counter() - t0
+    plan = nq.BatchPlan(mix)
+    plan.run()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        c2, st2 = plan.run()
+    t_replay = (time.perf_counter() - t0) / reps
+    plan.close()
+    same = [int(x) for x in counts] == per and [int(x) for x in c2] == per
+    out = {"instances": n, "per_launch_inst_per_s": n / t_per, "batch_inst_per_s": n / t_batch,
+           "batch_replay_inst_per_s": n / t_replay, "speedup": t_per / t_batch,
+           "speedup_replay": t_per / t_replay, "counts_identical": same, "per_launch_s": t_per,
+           "batch_s": t_batch, "batch_replay_s": t_replay, "batch_kernel_ms": st2["kernel_ms"],
+           "ntasks": int(st["ntasks"]), "solutions": int(sum(per)),
+           "workload": "512 instances, N uniform in [14,20], 2-6 fixed rows, seed 0x6E51"}
+    log(f"batched mix: per-launch {t_per:.3f}s  batch {t_batch:.3f}s  replay {t_replay:.3f}s  "
+        f"speedup {t_per / t_batch:.2f}x (replay {t_per / t_replay:.2f}x)  identical={same}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +106,8 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-wall", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
@@ -86,6 +122,9 @@ def main() -> int:
     N, rows = args.n, args.rows
     info = nq.device_info(0)
     log(f"device {info}")
+    if args.batch_only:
+        print(json.dumps({"batched_mix": batched_mix(nq, log)}), flush=True)
+        return 0
     t0 = time.perf_counter()
     tasks = nq.generate(N, rows)
     gen_s = time.perf_counter() - t0
@@ -170,6 +209,9 @@ def main() -> int:
             log(f"CLI: {m.group(0)} (process {outer:.2f} s)")
         else:
             log(f"CLI output not parsed: {p.stdout[-500:]} {p.stderr[-500:]}")
+
+    if not args.no_batch:
+        out["batched_mix"] = batched_mix(nq, log)
 
     if not args.no_cpu:
         def nodes_fn(n, r):

@@ -63,7 +63,7 @@ int nq_set_device(int device);        /* cudaSetDevice, n_queens_cuda.cu:149    
 /* Compute units, reported clock (kHz) and "name (gcnArch)" of a device. */
 int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t name_len);
 
-/* Fill cfg with defaults for (N, shallowest prefix depth d0): 64-thread workgroups, the
+/* Fill cfg with defaults for (N, shallowest prefix depth d0): 128-thread workgroups, the
  * smallest legal stack depth N-1-d0 (LDS per lane decides occupancy, so no frame is wasted),
  * chunk 1, and a persistent grid of every workgroup that fits on the device.  The reference's
  * compile-time presets only decide which (N, rows) pairs its CLI accepts; the CLI here keeps
@@ -105,6 +105,49 @@ int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_co
  * *total = the number of N-Queens solutions.  rows is used as given (the CLI applies the
  * reference's validation, src/main.cpp:10-29). */
 int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats);
+
+/* ---- batched instances (no reference counterpart) -------------------------------------
+ * An instance is a board of size N (1..27) whose first k rows (0 <= k <= N) hold queens at
+ * columns cols[0..k-1]; its count is the number of completions of that prefix — what the
+ * reference's n_queens(N, cur, left, right, sum) (src/n_queens.cpp:8-21) returns for the
+ * prefix masks — with no symmetry fold.  k = 0 is the empty board.
+ *
+ * A batch plan expands every instance on the host into tasks (a few more rows, chosen so that
+ * no task needs more than the stack depth), orders them largest-first, uploads them once, and
+ * captures "zero counts, reset cursor, persistent batched DFS" into a hipGraph; each run
+ * replays the graph on the plan's own stream.  Plans on different streams run concurrently. */
+typedef struct nq_instance {
+    int N;
+    int k;
+    int8_t cols[32];
+} nq_instance;
+
+typedef struct nq_batch_stats {
+    uint64_t ntasks;     /* device tasks after expansion                                */
+    uint64_t total;      /* sum of all instance counts                                  */
+    double expand_ms;    /* host validation + expansion (plan creation)                 */
+    double kernel_ms;    /* device time of the replayed graph (HIP events)              */
+    double total_ms;     /* whole call (nq_batch_count only)                            */
+} nq_batch_stats;
+
+typedef struct nq_batch_plan nq_batch_plan;
+
+/* Returns NQ_ERR_ARG (message names the instance) for N out of range, k > N, a column out of
+ * range or an attacking prefix.  cfg: block, chunk, waves_per_cu, grid as for nq_count;
+ * stack_depth (0 = 12 levels per task) bounds the per-task depth. */
+int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, nq_batch_plan **plan);
+int nq_batch_plan_launch(nq_batch_plan *plan);   /* enqueue one replay, asynchronous         */
+int nq_batch_plan_wait(nq_batch_plan *plan, uint64_t *counts, nq_batch_stats *stats);
+int nq_batch_plan_run(nq_batch_plan *plan, uint64_t *counts, nq_batch_stats *stats);
+int nq_batch_plan_destroy(nq_batch_plan *plan);
+
+/* One-shot: create, run once, destroy.  counts[i] = instance i's count. */
+int nq_batch_count(const nq_instance *inst, int n, const nq_config *cfg, uint64_t *counts,
+                   nq_batch_stats *stats);
+
+/* One instance through the plain (single-N) counting path: the per-instance launch the batch
+ * is measured against. */
+int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *count);
 
 #ifdef __cplusplus
 }

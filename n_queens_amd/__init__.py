@@ -35,6 +35,8 @@ NQ_ERR_STATE = -7
 EXPORTS = (
     "nq_device_count", "nq_last_error", "nq_set_device", "nq_device_info", "nq_config_default",
     "nq_generate", "nq_count", "nq_count_device", "nq_solve",
+    "nq_batch_plan_create", "nq_batch_plan_launch", "nq_batch_plan_wait", "nq_batch_plan_run",
+    "nq_batch_plan_destroy", "nq_batch_count", "nq_instance_count",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -56,6 +58,15 @@ class _Stats(ctypes.Structure):
     _fields_ = [("ntasks", ctypes.c_uint64), ("nodes", ctypes.c_uint64), ("gen_ms", ctypes.c_double),
                 ("h2d_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("busy_frac", ctypes.c_double), ("tail_ms", ctypes.c_double), ("waves", ctypes.c_uint64)]
+
+
+class _Instance(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("k", ctypes.c_int), ("cols", ctypes.c_int8 * 32)]
+
+
+class _BatchStats(ctypes.Structure):
+    _fields_ = [("ntasks", ctypes.c_uint64), ("total", ctypes.c_uint64), ("expand_ms", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
 
 @dataclass
@@ -94,6 +105,15 @@ def lib() -> ctypes.CDLL:
     L.nq_count_device.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(_Config),
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     L.nq_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Config), _u64p, ctypes.POINTER(_Stats)]
+    L.nq_batch_plan_create.argtypes = [ctypes.POINTER(_Instance), ctypes.c_int, ctypes.POINTER(_Config),
+                                       ctypes.POINTER(ctypes.c_void_p)]
+    L.nq_batch_plan_launch.argtypes = [ctypes.c_void_p]
+    L.nq_batch_plan_wait.argtypes = [ctypes.c_void_p, _u64p, ctypes.POINTER(_BatchStats)]
+    L.nq_batch_plan_run.argtypes = [ctypes.c_void_p, _u64p, ctypes.POINTER(_BatchStats)]
+    L.nq_batch_plan_destroy.argtypes = [ctypes.c_void_p]
+    L.nq_batch_count.argtypes = [ctypes.POINTER(_Instance), ctypes.c_int, ctypes.POINTER(_Config), _u64p,
+                                 ctypes.POINTER(_BatchStats)]
+    L.nq_instance_count.argtypes = [ctypes.POINTER(_Instance), ctypes.POINTER(_Config), _u64p]
     for name in EXPORTS:
         if name not in ("nq_last_error",):
             getattr(L, name).restype = ctypes.c_int
@@ -192,3 +212,74 @@ def cuda_n_queens(N: int, rows: int) -> int:
         return solve(N, rows)[0]
     except NQError:
         return -1
+
+
+# ---- batched instances -------------------------------------------------------------------
+
+def _instances(insts) -> ctypes.Array:
+    """[(N, [cols...]), ...] -> nq_instance array."""
+    arr = (_Instance * max(1, len(insts)))()
+    for i, (N, cols) in enumerate(insts):
+        arr[i].N = int(N)
+        arr[i].k = len(cols)
+        for j, c in enumerate(cols):
+            arr[i].cols[j] = int(c)
+    return arr
+
+
+def batch_count(insts, cfg: Config | None = None) -> tuple[np.ndarray, dict]:
+    """Counts of many (N, prefix-columns) instances in one batched launch (nq_batch_count)."""
+    arr = _instances(insts)
+    out = np.zeros(max(1, len(insts)), dtype=np.uint64)
+    st = _BatchStats()
+    cc = cfg._c() if cfg else None
+    _check(lib().nq_batch_count(arr, len(insts), ctypes.byref(cc) if cc else None, out.ctypes.data_as(_u64p),
+                                ctypes.byref(st)))
+    return out[: len(insts)], {"ntasks": st.ntasks, "total": st.total, "expand_ms": st.expand_ms,
+                               "kernel_ms": st.kernel_ms, "total_ms": st.total_ms}
+
+
+def instance_count(N: int, cols, cfg: Config | None = None) -> int:
+    """One instance through the plain per-instance launch (nq_instance_count)."""
+    arr = _instances([(N, cols)])
+    out = ctypes.c_uint64()
+    cc = cfg._c() if cfg else None
+    _check(lib().nq_instance_count(arr, ctypes.byref(cc) if cc else None, ctypes.byref(out)))
+    return int(out.value)
+
+
+class BatchPlan:
+    """Resident batch: expanded + uploaded once, launch sequence replayed from a hipGraph."""
+
+    def __init__(self, insts, cfg: Config | None = None):
+        self.n = len(insts)
+        arr = _instances(insts)
+        h = ctypes.c_void_p()
+        cc = cfg._c() if cfg else None
+        _check(lib().nq_batch_plan_create(arr, self.n, ctypes.byref(cc) if cc else None, ctypes.byref(h)))
+        self._h = h
+
+    def launch(self) -> None:
+        _check(lib().nq_batch_plan_launch(self._h))
+
+    def wait(self) -> tuple[np.ndarray, dict]:
+        out = np.zeros(max(1, self.n), dtype=np.uint64)
+        st = _BatchStats()
+        _check(lib().nq_batch_plan_wait(self._h, out.ctypes.data_as(_u64p), ctypes.byref(st)))
+        return out[: self.n], {"ntasks": st.ntasks, "total": st.total, "expand_ms": st.expand_ms,
+                               "kernel_ms": st.kernel_ms}
+
+    def run(self) -> tuple[np.ndarray, dict]:
+        self.launch()
+        return self.wait()
+
+    def close(self) -> None:
+        if self._h:
+            lib().nq_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

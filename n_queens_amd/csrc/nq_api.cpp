@@ -239,7 +239,7 @@ int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t 
 
 int nq_config_default(int N, int d0, nq_config *cfg) {
     if (!cfg) return fail(NQ_ERR_ARG, "cfg is NULL");
-    cfg->block = 64;
+    cfg->block = 128;
     cfg->stack_depth = std::max(1, frames_needed(N, d0));
     cfg->waves_per_cu = 0;
     cfg->chunk = 1;

@@ -1,0 +1,310 @@
+// Batched instances: many independent counting instances (different N, different fixed prefixes)
+// counted in one persistent launch over a device task queue, with the launch sequence captured
+// into a hipGraph and replayed.  No reference counterpart (the reference counts one whole board
+// per process, src/main.cpp:51); each instance's count is exactly what the reference's n_queens
+// (src/n_queens.cpp:8-21) returns for the instance's prefix masks.
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "nq.h"
+#include "nq_internal.h"
+#include "nq_kernels.h"
+
+namespace nq {
+
+namespace {
+
+struct Prefix {
+    uint32_t c, l, r;
+};
+
+// Compose the fixed rows exactly as the recursion does (cur|p, (left|p)<<1, (right|p)>>1).
+bool encode(const nq_instance &in, Prefix *out) {
+    if (in.N < 1 || in.N > MAX_N || in.k < 0 || in.k > in.N || in.k > 32) return false;
+    const uint32_t last = (1u << in.N) - 1u;
+    uint32_t c = 0, l = 0, r = 0;
+    for (int i = 0; i < in.k; ++i) {
+        const int col = in.cols[i];
+        if (col < 0 || col >= in.N) return false;
+        const uint32_t p = 1u << col;
+        if (!(p & last & ~(c | l | r))) return false;
+        c |= p;
+        l = (l | p) << 1;
+        r = (r | p) >> 1;
+    }
+    *out = {c, l, r};
+    return true;
+}
+
+// Rows to expand on the host so that no task needs more than `depth_target` levels.
+int split_rows(int N, int k, int depth_target) { return std::max(0, (N - k) - depth_target); }
+
+// All placements of `rows` more queens below (c, l, r), no symmetry restriction, appended as
+// {c, l, r, tag}.
+void expand(int N, uint32_t c, uint32_t l, uint32_t r, int rows, uint32_t tag, std::vector<uint4> &out) {
+    if (rows <= 0) {
+        out.push_back({c, l, r, tag});
+        return;
+    }
+    const uint32_t last = (1u << N) - 1u;
+    struct F {
+        uint32_t c, l, r, v;
+    };
+    F st[33];
+    int top = 0;
+    st[0] = {c, l, r, last & ~(c | l | r)};
+    while (top >= 0) {
+        F &f = st[top];
+        if (f.v == 0) {
+            --top;
+            continue;
+        }
+        const uint32_t p = f.v & (0u - f.v);
+        f.v ^= p;
+        const uint32_t c2 = f.c | p, l2 = (f.l | p) << 1, r2 = (f.r | p) >> 1;
+        if (top + 1 == rows) {
+            out.push_back({c2, l2, r2, tag});
+            continue;
+        }
+        ++top;
+        st[top] = {c2, l2, r2, last & ~(c2 | l2 | r2)};
+    }
+}
+
+}  // namespace
+
+// Validate instances and build the device task list (largest expected subtrees first).
+int build_batch_tasks(const nq_instance *inst, int n, int depth_target, std::vector<uint4> &tasks, int *max_frames) {
+    std::vector<Prefix> pre(n);
+    for (int i = 0; i < n; ++i) {
+        if (!encode(inst[i], &pre[i])) {
+            char buf[128];
+            snprintf(buf, sizeof buf, "instance %d: invalid N/k or attacking prefix", i);
+            return fail(NQ_ERR_ARG, buf);
+        }
+    }
+    // Order instances by remaining free rows after the split (then by N), descending: the
+    // biggest subtrees start first and small ones fill the tail.
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    auto rem_rows = [&](int i) {
+        const int N = inst[i].N, k = inst[i].k;
+        return (N - k) - split_rows(N, k, depth_target);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const int ra = rem_rows(a), rb = rem_rows(b);
+        return ra != rb ? ra > rb : inst[a].N > inst[b].N;
+    });
+    tasks.clear();
+    int frames = 1;
+    for (int i : order) {
+        const int N = inst[i].N, k = inst[i].k;
+        const int s = split_rows(N, k, depth_target);
+        expand(N, pre[i].c, pre[i].l, pre[i].r, s, (static_cast<uint32_t>(i) << 5) | static_cast<uint32_t>(N), tasks);
+        const int d0 = k + s;
+        if (d0 + 1 < N) frames = std::max(frames, N - 1 - d0);
+    }
+    *max_frames = frames;
+    return NQ_OK;
+}
+
+int instance_tasks(const nq_instance &in, int depth_target, std::vector<uint32_t> &tasks, int *d0) {
+    Prefix p;
+    if (!encode(in, &p)) return fail(NQ_ERR_ARG, "invalid N/k or attacking prefix");
+    const int s = split_rows(in.N, in.k, depth_target);
+    std::vector<uint4> t;
+    expand(in.N, p.c, p.l, p.r, s, 0, t);
+    tasks.resize(3 * t.size());
+    for (size_t i = 0; i < t.size(); ++i) {
+        tasks[3 * i] = t[i].x;
+        tasks[3 * i + 1] = t[i].y;
+        tasks[3 * i + 2] = t[i].z;
+    }
+    *d0 = in.k + s;
+    return NQ_OK;
+}
+
+}  // namespace nq
+
+using namespace nq;
+
+#define NQ_HIP(call)                                             \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return fail_hip(e_, #call);        \
+    } while (0)
+
+struct nq_batch_plan {
+    int device = 0;
+    uint64_t ninst = 0, ntasks = 0;
+    uint4 *d_tasks = nullptr;
+    uint64_t *d_counts = nullptr;
+    DevState *d_state = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    nq_config cfg{};
+    double expand_ms = 0;
+};
+
+static void plan_free(nq_batch_plan *p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->d_tasks) (void)hipFree(p->d_tasks);
+    if (p->d_counts) (void)hipFree(p->d_counts);
+    if (p->d_state) (void)hipFree(p->d_state);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+extern "C" {
+
+int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, nq_batch_plan **out) {
+    if (!out) return fail(NQ_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n < 0 || (n > 0 && !inst)) return fail(NQ_ERR_ARG, "bad instance array");
+    const double t0 = now_ms();
+    nq_config c;
+    nq_config_default(MAX_N, 0, &c);
+    c.stack_depth = 0;
+    if (cfg) {
+        if (cfg->block) c.block = cfg->block;
+        if (cfg->waves_per_cu) c.waves_per_cu = cfg->waves_per_cu;
+        if (cfg->chunk) c.chunk = cfg->chunk;
+        if (cfg->grid) c.grid = cfg->grid;
+        if (cfg->stack_depth) c.stack_depth = cfg->stack_depth;
+    }
+    if (c.block < 64 || c.block > MAX_BLOCK || c.block % 64) return fail(NQ_ERR_ARG, "block must be 64..256, multiple of 64");
+    if (c.chunk < 1 || c.chunk > 4096) return fail(NQ_ERR_ARG, "chunk must be 1..4096");
+    // Per-task depth target: the stack depth + 1 when given, else 12 free rows (11 frames).
+    const int depth_target = c.stack_depth ? c.stack_depth + 1 : 12;
+    std::vector<uint4> tasks;
+    int frames = 1;
+    int rc = build_batch_tasks(inst, n, depth_target, tasks, &frames);
+    if (rc) return rc;
+    if (!c.stack_depth) c.stack_depth = frames;
+    if (c.stack_depth < frames || c.stack_depth > 32) return fail(NQ_ERR_DEPTH, "stack depth out of range for the batch");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(NQ_ERR_NO_DEVICE, "Failed to find any gpu!");
+
+    auto *p = new nq_batch_plan();
+    p->ninst = static_cast<uint64_t>(n);
+    p->ntasks = tasks.size();
+    p->expand_ms = now_ms() - t0;
+    hipError_t e = hipGetDevice(&p->device);
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&p->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&p->ev1);
+    if (e == hipSuccess) e = hipMalloc(&p->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc(&p->d_counts, std::max<uint64_t>(1, p->ninst) * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&p->d_state, sizeof(DevState));
+    if (e == hipSuccess && !tasks.empty())
+        e = hipMemcpyAsync(p->d_tasks, tasks.data(), tasks.size() * sizeof(uint4), hipMemcpyHostToDevice, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) {
+        plan_free(p);
+        return fail_hip(e, "batch plan allocation/upload");
+    }
+    if (c.grid == 0) {
+        int bpc = 0;
+        e = batch_occupancy(c.block, c.stack_depth, &bpc);
+        if (e != hipSuccess || bpc <= 0) {
+            plan_free(p);
+            return e != hipSuccess ? fail_hip(e, "batch_occupancy") : fail(NQ_ERR_ARG, "batch does not fit on a CU");
+        }
+        if (c.waves_per_cu > 0) bpc = std::max(1, std::min(bpc, c.waves_per_cu / (c.block / 64)));
+        c.grid = cus * bpc;
+        const uint64_t need = (p->ntasks + c.block - 1) / c.block;
+        if (static_cast<uint64_t>(c.grid) > need) c.grid = static_cast<int>(std::max<uint64_t>(1, need));
+    }
+    p->cfg = c;
+    // Capture zero-counts + reset + batched DFS into a graph (kernel nodes only).
+    BatchArgs a{p->d_tasks, p->ntasks, p->d_counts, std::max<uint64_t>(1, p->ninst), c.block, c.grid, c.stack_depth,
+                c.chunk, p->d_state};
+    e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t le = launch_batch(a, p->stream);
+        e = hipStreamEndCapture(p->stream, &p->graph);
+        if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        plan_free(p);
+        return fail_hip(e, "batch graph capture");
+    }
+    *out = p;
+    return NQ_OK;
+}
+
+int nq_batch_plan_launch(nq_batch_plan *p) {
+    if (!p) return fail(NQ_ERR_ARG, "plan is NULL");
+    NQ_HIP(hipEventRecord(p->ev0, p->stream));
+    NQ_HIP(hipGraphLaunch(p->exec, p->stream));
+    NQ_HIP(hipEventRecord(p->ev1, p->stream));
+    return NQ_OK;
+}
+
+int nq_batch_plan_wait(nq_batch_plan *p, uint64_t *counts, nq_batch_stats *stats) {
+    if (!p) return fail(NQ_ERR_ARG, "plan is NULL");
+    DevState st;
+    if (counts && p->ninst)
+        NQ_HIP(hipMemcpyAsync(counts, p->d_counts, p->ninst * sizeof(uint64_t), hipMemcpyDeviceToHost, p->stream));
+    NQ_HIP(hipMemcpyAsync(&st, p->d_state, sizeof st, hipMemcpyDeviceToHost, p->stream));
+    NQ_HIP(hipStreamSynchronize(p->stream));
+    if (st.error) return fail(NQ_ERR_DEPTH, "a batched task needed more stack frames than configured");
+    if (stats) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
+        stats->ntasks = p->ntasks;
+        stats->expand_ms = p->expand_ms;
+        stats->kernel_ms = ms;
+        stats->total = st.total;
+    }
+    return NQ_OK;
+}
+
+int nq_batch_plan_run(nq_batch_plan *p, uint64_t *counts, nq_batch_stats *stats) {
+    int rc = nq_batch_plan_launch(p);
+    if (rc) return rc;
+    return nq_batch_plan_wait(p, counts, stats);
+}
+
+int nq_batch_plan_destroy(nq_batch_plan *p) {
+    plan_free(p);
+    return NQ_OK;
+}
+
+int nq_batch_count(const nq_instance *inst, int n, const nq_config *cfg, uint64_t *counts, nq_batch_stats *stats) {
+    const double t0 = now_ms();
+    nq_batch_plan *p = nullptr;
+    int rc = nq_batch_plan_create(inst, n, cfg, &p);
+    if (rc) return rc;
+    rc = nq_batch_plan_run(p, counts, stats);
+    plan_free(p);
+    if (!rc && stats) stats->total_ms = now_ms() - t0;
+    return rc;
+}
+
+int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *count) {
+    if (!inst || !count) return fail(NQ_ERR_ARG, "NULL argument");
+    std::vector<uint32_t> tasks;
+    int d0 = 0;
+    const int depth_target = (cfg && cfg->stack_depth) ? cfg->stack_depth + 1 : 12;
+    int rc = instance_tasks(*inst, depth_target, tasks, &d0);
+    if (rc) return rc;
+    return nq_count(inst->N, tasks.data(), tasks.size() / 3, cfg, count, nullptr, nullptr, nullptr);
+}
+
+}  // extern "C"

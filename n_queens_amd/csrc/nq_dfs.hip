@@ -34,7 +34,7 @@ namespace nq {
 namespace {
 
 constexpr uint32_t DEPTH_ONE = 1u << DEPTH_SHIFT;
-constexpr uint32_t FLUSH_EVERY = 1u << 30;  // hot-loop trips (2 iterations each) between folds
+constexpr uint32_t FLUSH_EVERY = 1u << 29;  // hot-loop trips (4 iterations each) between folds
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
@@ -127,14 +127,16 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
     "s_andn2_b64 exec, %[tmp], %[m1]\n\t"                                                     \
     "s_waitcnt lgkmcnt(0)\n\t"
 
-// The whole hot loop: two iterations per trip, leaving when some lane becomes done or after
-// [iters] trips (the caller folds the 32-bit counters then).
-#define NQ_HOT_ASM(NODES_INSN)                                                                \
+// The whole hot loop: four iterations per trip, leaving when some lane becomes done or after
+// [iters] trips (the caller folds the 32-bit counters then).  BC is the constraint of the board
+// mask and last-row threshold: "s" when the launch has one N, "v" when N varies per lane.
+#define NQ_HOT_ASM(NODES_INSN, BC)                                                            \
     __asm__ volatile(                                                                         \
         "s_mov_b64 %[save], exec\n\t"                                                         \
         "s_mov_b64 exec, %[live]\n\t"                                                         \
         "s_mov_b64 %[done], 0\n"                                                              \
-        "nq_hot_%=:\n\t" NQ_ITER(NODES_INSN) NQ_ITER(NODES_INSN)                              \
+        "nq_hot_%=:\n\t" NQ_ITER(NODES_INSN) NQ_ITER(NODES_INSN) NQ_ITER(NODES_INSN)         \
+        NQ_ITER(NODES_INSN)                                                                   \
         "s_cmp_lg_u64 %[done], 0\n\t"                                                         \
         "s_cbranch_scc1 nq_exit_%=\n\t"                                                       \
         "s_sub_u32 %[iters], %[iters], 1\n\t"                                                 \
@@ -146,7 +148,7 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
           [nd] "+v"(nd), [iters] "+s"(iters), [t] "=&v"(t), [p] "=&v"(p), [m1] "=&s"(m1),       \
           [m2] "=&s"(m2), [tmp] "=&s"(tmp), [save] "=&s"(save), [done] "=&s"(done)             \
         : [live] "s"(live), [base] "v"(base), [stride] "s"(stride), [one] "s"(DEPTH_ONE),     \
-          [last] "s"(last), [leaf] "s"(leaf)                                                  \
+          [last] BC(last), [leaf] BC(leaf)                                                    \
         : "memory", "scc")
 
 template <bool NODES, bool PER_TASK>
@@ -222,14 +224,14 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
         const uint64_t live = __ballot(has);
         if (live == 0) break;
         // Hot loop until a lane needs a new task; the trip bound keeps the 32-bit per-lane
-        // counters from wrapping (at most 2 * FLUSH_EVERY hits between folds).
+        // counters from wrapping (at most 4 * FLUSH_EVERY hits between folds).
         {
             uint32_t t, p, iters = FLUSH_EVERY;
             uint64_t m1, m2, tmp, save, done;
             if (NODES)
-                NQ_HOT_ASM("v_add_u32 %[nd], 1, %[nd]\n\t");
+                NQ_HOT_ASM("v_add_u32 %[nd], 1, %[nd]\n\t", "s");
             else
-                NQ_HOT_ASM("");
+                NQ_HOT_ASM("", "s");
         }
         tsum += cnt;
         if (NODES) nodes += static_cast<uint64_t>(nd) - cnt;
@@ -260,6 +262,96 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
         atomicAdd(&st->total, s);
         if (NODES) atomicAdd(&st->nodes, n);
     }
+}
+
+// Batched instances: tasks are {cur, left, right, inst << 5 | N}; N (and with it the board
+// mask and the last-row threshold) varies per lane, and every finished task adds its count to
+// counts[inst] with one 64-bit atomic.  Same iteration as dfs_kernel.
+__global__ void __launch_bounds__(MAX_BLOCK)
+batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, uint32_t stack_depth,
+             DevState *__restrict__ st, unsigned long long *__restrict__ counts) {
+    extern __shared__ uint4 stack[];
+    __shared__ unsigned long long wq[MAX_WAVES_PER_BLOCK][2];
+    __shared__ unsigned long long red[MAX_WAVES_PER_BLOCK];
+
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        wq[wave][0] = 0;
+        wq[wave][1] = 0;
+    }
+    const uint32_t stride = blockDim.x * static_cast<uint32_t>(sizeof(uint4));
+    const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(stack + threadIdx.x));
+    uint32_t sp = base;
+    uint32_t c = 0, l = 0, r = 0, v = 0, cnt = 0, nd = 0;
+    uint32_t last = 0, leaf = 0, inst = 0;
+    uint64_t tsum = 0, acc = 0;
+    bool has = true, own = false;
+
+    for (;;) {
+        if (has && v == 0) {
+            if (own) {
+                atomicAdd(&counts[inst], static_cast<unsigned long long>(tsum));
+                acc += tsum;
+            }
+            tsum = 0;
+            own = false;
+            has = false;
+            for (;;) {
+                const uint64_t t = claim(wq[wave], &st->cursor, chunk);
+                if (t >= ntasks) break;
+                const uint4 w = tasks[t];
+                const uint32_t N = w.w & 31u;
+                inst = w.w >> 5;
+                last = (1u << N) - 1u;
+                leaf = (N - 1u) << DEPTH_SHIFT;
+                const uint32_t v0 = last & ~(w.x | w.y | w.z);
+                const uint32_t d0 = __popc(w.x);
+                if (d0 + 1u >= N || v0 == 0) {
+                    const uint64_t direct =
+                        d0 >= N ? (w.x == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
+                    if (direct) {
+                        atomicAdd(&counts[inst], static_cast<unsigned long long>(direct));
+                        acc += direct;
+                    }
+                    continue;
+                }
+                if (N - 1u - d0 > stack_depth) {
+                    atomicOr(&st->error, 1ull);
+                    continue;
+                }
+                c = w.x | (d0 << DEPTH_SHIFT);
+                l = w.y;
+                r = w.z;
+                v = v0;
+                has = true;
+                own = true;
+                break;
+            }
+        }
+        const uint64_t live = __ballot(has);
+        if (live == 0) break;
+        {
+            uint32_t t, p, iters = FLUSH_EVERY;
+            uint64_t m1, m2, tmp, save, done;
+            NQ_HOT_ASM("", "v");
+        }
+        tsum += cnt;
+        cnt = 0;
+    }
+    acc = wave_sum_u64(acc);
+    if ((threadIdx.x & 63u) == 0) red[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) s += red[w];
+        atomicAdd(&st->total, s);
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0;
 }
 
 __global__ void reset_state_kernel(DevState *st) {
@@ -295,6 +387,30 @@ hipError_t allow_lds(const void *k, size_t lds) {
 }
 
 }  // namespace
+
+hipError_t batch_occupancy(int block, int stack_depth, int *blocks_per_cu) {
+    const void *k = reinterpret_cast<const void *>(&batch_kernel);
+    const size_t lds = dfs_lds_bytes(block, stack_depth);
+    hipError_t e = allow_lds(k, lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, block, lds);
+}
+
+hipError_t launch_batch(const BatchArgs &a, hipStream_t stream) {
+    zero_u64_kernel<<<64, 256, 0, stream>>>(reinterpret_cast<unsigned long long *>(a.counts), a.ninst);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    reset_state_kernel<<<1, 64, 0, stream>>>(a.state);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t lds = dfs_lds_bytes(a.block, a.stack_depth);
+    e = allow_lds(reinterpret_cast<const void *>(&batch_kernel), lds);
+    if (e != hipSuccess) return e;
+    batch_kernel<<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, static_cast<uint32_t>(a.chunk),
+                                                  static_cast<uint32_t>(a.stack_depth), a.state,
+                                                  reinterpret_cast<unsigned long long *>(a.counts));
+    return hipGetLastError();
+}
 
 size_t dfs_lds_bytes(int block, int stack_depth) {
     return static_cast<size_t>(block) * static_cast<size_t>(stack_depth) * sizeof(uint4);

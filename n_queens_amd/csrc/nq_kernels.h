@@ -37,6 +37,19 @@ struct LaunchArgs {
     uint64_t *trace = nullptr;  // nullable: per wave {start, end} wall-clock ticks (grid * block/64 * 2)
 };
 
+struct BatchArgs {
+    const uint4 *tasks;  // {cur, left, right, inst << 5 | N}
+    uint64_t ntasks;
+    uint64_t *counts;    // per instance, zeroed by the launch
+    uint64_t ninst;
+    int block, grid, stack_depth, chunk;
+    DevState *state;
+};
+
+// Enqueue: zero counts, reset state, batched DFS.  Capturable into a hipGraph (kernels only).
+hipError_t launch_batch(const BatchArgs &a, hipStream_t stream);
+hipError_t batch_occupancy(int block, int stack_depth, int *blocks_per_cu);
+
 // Enqueue: reset state, DFS, (optional) fold into result.  Returns a hipError_t.
 hipError_t launch_count(const LaunchArgs &a, hipStream_t stream);
 
