@@ -97,6 +97,71 @@ def batched_mix(nq, log) -> dict:
     return out
 
 
+def serving(nq, log, n_req: int = 400, util: float = 0.7) -> dict:
+    """Poisson arrivals (seeded) of requests with N in [12,19] and a 0-5 row prefix, at a rate
+    set to `util` of the measured service capacity; p50/p99 latency and solutions/s; every
+    served count checked against the same request counted alone."""
+    import threading
+    from n_queens_amd import workloads
+    reqs0 = workloads.poisson_requests(n_req, lam=1.0)
+    alone = [nq.instance_count(N, cols) for _, N, cols in reqs0]
+    busy = [0]
+
+    def submit(srv, N, cols):
+        # The server refuses work beyond its pending-task budget (NQ_ERR_BUSY): back off, retry.
+        while True:
+            try:
+                return srv.submit(N, cols)
+            except nq.NQError as e:
+                if e.status != nq.NQ_ERR_BUSY:
+                    raise
+                busy[0] += 1
+                time.sleep(0.0005)
+
+    with nq.Server() as srv:
+        # capacity: all requests offered at once
+        t0 = time.perf_counter()
+        tickets = [submit(srv, N, cols) for _, N, cols in reqs0]
+        burst = [srv.wait(t, timeout_ms=120000)[0] for t in tickets]
+        log(f"serving capacity probe done: {srv.stats()}")
+        cap_s = time.perf_counter() - t0
+    busy_burst, busy[0] = busy[0], 0
+    capacity = n_req / cap_s
+    lam = util * capacity
+    reqs = [(at / lam, N, cols) for at, N, cols in reqs0]  # same requests, arrival times scaled
+    tickets = [None] * n_req
+    with nq.Server() as srv:
+        t0 = time.perf_counter()
+
+        def feed():
+            for i, (at, N, cols) in enumerate(reqs):
+                d = at - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = submit(srv, N, cols)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        res = [srv.wait(t, timeout_ms=120000) for t in tickets]
+        span = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([x["latency_ms"] for _, x in res])
+    served = [c for c, _ in res]
+    same = served == alone and burst == alone
+    out = {"requests": n_req, "capacity_req_per_s": capacity, "lambda_req_per_s": lam, "target_util": util,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()), "solutions_per_s": sum(served) / span, "req_per_s": n_req / span,
+           "batches": int(st["batches"]), "counts_identical_alone_vs_mix": same, "span_s": span,
+           "busy_retries": busy[0], "busy_retries_capacity_probe": busy_burst,
+           "device_busy_ms": st["device_ms"],
+           "workload": "Poisson arrivals, N uniform in [12,19], 0-5 fixed rows, seed 0x5E7E"}
+    log(f"serving: capacity {capacity:.1f} req/s, lambda {lam:.1f}/s, p50 {out['p50_ms']:.2f} ms, "
+        f"p99 {out['p99_ms']:.2f} ms, {out['solutions_per_s']:.3e} solutions/s, batches {st['batches']}, "
+        f"identical={same}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,12 +173,15 @@ def main() -> int:
     ap.add_argument("--no-wall", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
               f"path (asked for --gpus {args.gpus}).", file=sys.stderr)
         return 2
 
+    global np
     import numpy as np
     import torch
 
@@ -124,6 +192,9 @@ def main() -> int:
     log(f"device {info}")
     if args.batch_only:
         print(json.dumps({"batched_mix": batched_mix(nq, log)}), flush=True)
+        return 0
+    if args.serve_only:
+        print(json.dumps({"serving": serving(nq, log)}), flush=True)
         return 0
     t0 = time.perf_counter()
     tasks = nq.generate(N, rows)
@@ -212,6 +283,8 @@ def main() -> int:
 
     if not args.no_batch:
         out["batched_mix"] = batched_mix(nq, log)
+    if not args.no_serve:
+        out["serving"] = serving(nq, log)
 
     if not args.no_cpu:
         def nodes_fn(n, r):

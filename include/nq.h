@@ -149,6 +149,52 @@ int nq_batch_count(const nq_instance *inst, int n, const nq_config *cfg, uint64_
  * is measured against. */
 int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *count);
 
+/* ---- serving with continuous batching (no reference counterpart) ---------------------
+ * A server keeps `slots` device-resident task queues (tasks, per-request counters, cursor,
+ * pinned result buffer, stream).  nq_server_submit validates a request, expands it into tasks
+ * on the caller's thread and admits it while the pending work fits max_pending_tasks (else
+ * NQ_ERR_BUSY; NQ_ERR_BUSY also for one request larger than slot_tasks).  A request is
+ * expanded until no task keeps more than depth_target free rows.  A scheduler thread
+ * cuts a batch into each free slot: pending requests by least estimated remaining work (aged
+ * by waiting time), each contributing tasks until the batch's estimated work (batch_work DFS
+ * nodes) is spent, so large requests are served in slices over consecutive batches while small
+ * ones join the next batch; batches run on their slot's stream and overlap on the device.  A
+ * request's count equals nq_instance_count's for the same instance.  Calls on one server are
+ * thread-safe. */
+typedef struct nq_server_config {
+    int slots;                   /* concurrent batches in flight (default 3)               */
+    uint64_t slot_tasks;         /* task capacity of one slot (default 4M)                 */
+    int slot_requests;           /* requests per batch (default 4096)                      */
+    uint64_t max_pending_tasks;  /* admission budget of queued tasks (default 16M)         */
+    int depth_target;            /* most free rows a task keeps (default 13): bounds both  */
+                                 /* the LDS stack and one task's run time on its lane      */
+    int block;                   /* workgroup size (default 128)                           */
+    int chunk;                   /* tasks per cursor grab (default 4)                      */
+    double batch_work;           /* estimated DFS nodes per batch (default 2e10, ~20 ms)   */
+} nq_server_config;
+
+typedef struct nq_request_stats {
+    double queue_ms;             /* submit -> its batch launched                           */
+    double latency_ms;           /* submit -> result ready on the host                     */
+    uint64_t ntasks;             /* tasks the request was expanded into                    */
+} nq_request_stats;
+
+typedef struct nq_server nq_server;
+
+int nq_server_config_default(nq_server_config *cfg);
+int nq_server_open(const nq_server_config *cfg, nq_server **server);
+int nq_server_submit(nq_server *server, const nq_instance *inst, uint64_t *ticket);
+/* NQ_OK and the count when done (the ticket is then released), NQ_ERR_STATE while pending. */
+int nq_server_poll(nq_server *server, uint64_t ticket, uint64_t *count, nq_request_stats *stats);
+/* Block until done or timeout_ms (< 0: no timeout); NQ_ERR_STATE on timeout. */
+int nq_server_wait(nq_server *server, uint64_t ticket, int timeout_ms, uint64_t *count,
+                   nq_request_stats *stats);
+int nq_server_stats(nq_server *server, uint64_t *completed, uint64_t *batches, uint64_t *solutions);
+/* Summed batch residency (launch -> results observed on the host) and time since open. */
+int nq_server_timing(nq_server *server, double *device_ms, double *open_ms);
+/* Finish every admitted request, stop the scheduler and free the device queues. */
+int nq_server_close(nq_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,8 @@ EXPORTS = (
     "nq_generate", "nq_count", "nq_count_device", "nq_solve",
     "nq_batch_plan_create", "nq_batch_plan_launch", "nq_batch_plan_wait", "nq_batch_plan_run",
     "nq_batch_plan_destroy", "nq_batch_count", "nq_instance_count",
+    "nq_server_config_default", "nq_server_open", "nq_server_submit", "nq_server_poll", "nq_server_wait",
+    "nq_server_stats", "nq_server_timing", "nq_server_close",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -67,6 +69,16 @@ class _Instance(ctypes.Structure):
 class _BatchStats(ctypes.Structure):
     _fields_ = [("ntasks", ctypes.c_uint64), ("total", ctypes.c_uint64), ("expand_ms", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("slot_tasks", ctypes.c_uint64), ("slot_requests", ctypes.c_int),
+                ("max_pending_tasks", ctypes.c_uint64), ("depth_target", ctypes.c_int), ("block", ctypes.c_int),
+                ("chunk", ctypes.c_int), ("batch_work", ctypes.c_double)]
+
+
+class _RequestStats(ctypes.Structure):
+    _fields_ = [("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("ntasks", ctypes.c_uint64)]
 
 
 @dataclass
@@ -114,6 +126,14 @@ def lib() -> ctypes.CDLL:
     L.nq_batch_count.argtypes = [ctypes.POINTER(_Instance), ctypes.c_int, ctypes.POINTER(_Config), _u64p,
                                  ctypes.POINTER(_BatchStats)]
     L.nq_instance_count.argtypes = [ctypes.POINTER(_Instance), ctypes.POINTER(_Config), _u64p]
+    L.nq_server_config_default.argtypes = [ctypes.POINTER(_ServerConfig)]
+    L.nq_server_open.argtypes = [ctypes.POINTER(_ServerConfig), ctypes.POINTER(ctypes.c_void_p)]
+    L.nq_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Instance), _u64p]
+    L.nq_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u64p, ctypes.POINTER(_RequestStats)]
+    L.nq_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, _u64p, ctypes.POINTER(_RequestStats)]
+    L.nq_server_stats.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p]
+    L.nq_server_close.argtypes = [ctypes.c_void_p]
+    L.nq_server_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     for name in EXPORTS:
         if name not in ("nq_last_error",):
             getattr(L, name).restype = ctypes.c_int
@@ -283,3 +303,59 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+# ---- serving -------------------------------------------------------------------------------
+
+class Server:
+    """Continuous-batching request server (nq_server_*)."""
+
+    def __init__(self, **cfg):
+        c = _ServerConfig()
+        _check(lib().nq_server_config_default(ctypes.byref(c)))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        h = ctypes.c_void_p()
+        _check(lib().nq_server_open(ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, N: int, cols) -> int:
+        arr = _instances([(N, cols)])
+        t = ctypes.c_uint64()
+        _check(lib().nq_server_submit(self._h, arr, ctypes.byref(t)))
+        return int(t.value)
+
+    def poll(self, ticket: int):
+        """(count, stats) when done, None while pending."""
+        c = ctypes.c_uint64()
+        st = _RequestStats()
+        rc = lib().nq_server_poll(self._h, ticket, ctypes.byref(c), ctypes.byref(st))
+        if rc == NQ_ERR_STATE:
+            return None
+        _check(rc)
+        return int(c.value), {"queue_ms": st.queue_ms, "latency_ms": st.latency_ms, "ntasks": int(st.ntasks)}
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        c = ctypes.c_uint64()
+        st = _RequestStats()
+        _check(lib().nq_server_wait(self._h, ticket, timeout_ms, ctypes.byref(c), ctypes.byref(st)))
+        return int(c.value), {"queue_ms": st.queue_ms, "latency_ms": st.latency_ms, "ntasks": int(st.ntasks)}
+
+    def stats(self) -> dict:
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().nq_server_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        d, o = ctypes.c_double(), ctypes.c_double()
+        _check(lib().nq_server_timing(self._h, ctypes.byref(d), ctypes.byref(o)))
+        return {"completed": a.value, "batches": b.value, "solutions": c.value, "device_ms": d.value,
+                "open_ms": o.value}
+
+    def close(self) -> None:
+        if self._h:
+            h, self._h = self._h, None
+            _check(lib().nq_server_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -79,6 +79,34 @@ void expand(int N, uint32_t c, uint32_t l, uint32_t r, int rows, uint32_t tag, s
 
 }  // namespace
 
+// Tasks of one instance, tagged idx << 6 | twice << 5 | N; returns their prefix depth.  An empty
+// board (k = 0, N >= 4) is split with the reference's mirror-symmetric generator
+// (src/n_queens.cpp:74-113) and its tasks carry weight 2, halving its work; every other prefix
+// is expanded plainly.
+int append_instance_tasks(int N, int k, uint32_t c, uint32_t l, uint32_t r, int depth_target, uint32_t idx,
+                          std::vector<uint4> &out) {
+    const int s = split_rows(N, k, depth_target);
+    if (k == 0 && N >= 4) {
+        const int rows = std::max(2, s);
+        std::vector<uint32_t> t;
+        generate_tasks(N, rows, t);
+        const uint32_t tag = (idx << 6) | (1u << 5) | static_cast<uint32_t>(N);
+        for (size_t i = 0; i + 2 < t.size(); i += 3) out.push_back({t[i], t[i + 1], t[i + 2], tag});
+        return rows;
+    }
+    expand(N, c, l, r, s, (idx << 6) | static_cast<uint32_t>(N), out);
+    return k + s;
+}
+
+bool encode_instance(const nq_instance &in, uint32_t *c, uint32_t *l, uint32_t *r) {
+    Prefix p;
+    if (!encode(in, &p)) return false;
+    *c = p.c;
+    *l = p.l;
+    *r = p.r;
+    return true;
+}
+
 // Validate instances and build the device task list (largest expected subtrees first).
 int build_batch_tasks(const nq_instance *inst, int n, int depth_target, std::vector<uint4> &tasks, int *max_frames) {
     std::vector<Prefix> pre(n);
@@ -105,28 +133,26 @@ int build_batch_tasks(const nq_instance *inst, int n, int depth_target, std::vec
     int frames = 1;
     for (int i : order) {
         const int N = inst[i].N, k = inst[i].k;
-        const int s = split_rows(N, k, depth_target);
-        expand(N, pre[i].c, pre[i].l, pre[i].r, s, (static_cast<uint32_t>(i) << 5) | static_cast<uint32_t>(N), tasks);
-        const int d0 = k + s;
+        const int d0 = append_instance_tasks(N, k, pre[i].c, pre[i].l, pre[i].r, depth_target, static_cast<uint32_t>(i),
+                                             tasks);
         if (d0 + 1 < N) frames = std::max(frames, N - 1 - d0);
     }
     *max_frames = frames;
     return NQ_OK;
 }
 
-int instance_tasks(const nq_instance &in, int depth_target, std::vector<uint32_t> &tasks, int *d0) {
+int instance_tasks(const nq_instance &in, int depth_target, std::vector<uint32_t> &tasks, int *weight) {
     Prefix p;
     if (!encode(in, &p)) return fail(NQ_ERR_ARG, "invalid N/k or attacking prefix");
-    const int s = split_rows(in.N, in.k, depth_target);
     std::vector<uint4> t;
-    expand(in.N, p.c, p.l, p.r, s, 0, t);
+    append_instance_tasks(in.N, in.k, p.c, p.l, p.r, depth_target, 0, t);
     tasks.resize(3 * t.size());
     for (size_t i = 0; i < t.size(); ++i) {
         tasks[3 * i] = t[i].x;
         tasks[3 * i + 1] = t[i].y;
         tasks[3 * i + 2] = t[i].z;
     }
-    *d0 = in.k + s;
+    *weight = (!t.empty() && (t[0].w & (1u << 5))) ? 2 : 1;
     return NQ_OK;
 }
 
@@ -300,11 +326,15 @@ int nq_batch_count(const nq_instance *inst, int n, const nq_config *cfg, uint64_
 int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *count) {
     if (!inst || !count) return fail(NQ_ERR_ARG, "NULL argument");
     std::vector<uint32_t> tasks;
-    int d0 = 0;
+    int weight = 1;
     const int depth_target = (cfg && cfg->stack_depth) ? cfg->stack_depth + 1 : 12;
-    int rc = instance_tasks(*inst, depth_target, tasks, &d0);
+    int rc = instance_tasks(*inst, depth_target, tasks, &weight);
     if (rc) return rc;
-    return nq_count(inst->N, tasks.data(), tasks.size() / 3, cfg, count, nullptr, nullptr, nullptr);
+    uint64_t sum = 0;
+    rc = nq_count(inst->N, tasks.data(), tasks.size() / 3, cfg, &sum, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    *count = sum * static_cast<uint64_t>(weight);
+    return NQ_OK;
 }
 
 }  // extern "C"

@@ -264,9 +264,10 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
     }
 }
 
-// Batched instances: tasks are {cur, left, right, inst << 5 | N}; N (and with it the board
-// mask and the last-row threshold) varies per lane, and every finished task adds its count to
-// counts[inst] with one 64-bit atomic.  Same iteration as dfs_kernel.
+// Batched instances: tasks are {cur, left, right, inst << 6 | twice << 5 | N}; N (and with it
+// the board mask and the last-row threshold) varies per lane, and every finished task adds its
+// count (doubled when `twice` is set: a mirror-symmetric half of an empty board) to counts[inst]
+// with one 64-bit atomic.  Same iteration as dfs_kernel.
 __global__ void __launch_bounds__(MAX_BLOCK)
 batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, uint32_t stack_depth,
              DevState *__restrict__ st, unsigned long long *__restrict__ counts) {
@@ -283,15 +284,15 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
     const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(stack + threadIdx.x));
     uint32_t sp = base;
     uint32_t c = 0, l = 0, r = 0, v = 0, cnt = 0, nd = 0;
-    uint32_t last = 0, leaf = 0, inst = 0;
+    uint32_t last = 0, leaf = 0, inst = 0, weight = 1;
     uint64_t tsum = 0, acc = 0;
     bool has = true, own = false;
 
     for (;;) {
         if (has && v == 0) {
             if (own) {
-                atomicAdd(&counts[inst], static_cast<unsigned long long>(tsum));
-                acc += tsum;
+                atomicAdd(&counts[inst], static_cast<unsigned long long>(tsum * weight));
+                acc += tsum * weight;
             }
             tsum = 0;
             own = false;
@@ -301,7 +302,8 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
                 if (t >= ntasks) break;
                 const uint4 w = tasks[t];
                 const uint32_t N = w.w & 31u;
-                inst = w.w >> 5;
+                weight = 1u + ((w.w >> 5) & 1u);
+                inst = w.w >> 6;
                 last = (1u << N) - 1u;
                 leaf = (N - 1u) << DEPTH_SHIFT;
                 const uint32_t v0 = last & ~(w.x | w.y | w.z);
@@ -310,8 +312,8 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
                     const uint64_t direct =
                         d0 >= N ? (w.x == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
                     if (direct) {
-                        atomicAdd(&counts[inst], static_cast<unsigned long long>(direct));
-                        acc += direct;
+                        atomicAdd(&counts[inst], static_cast<unsigned long long>(direct * weight));
+                        acc += direct * weight;
                     }
                     continue;
                 }

@@ -1,0 +1,95 @@
+"""Serving with continuous batching: admission errors, results alone vs in the mix, Poisson
+arrivals, empty-board requests."""
+import os
+import subprocess
+import sys
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import n_queens_amd as nq
+import oracle
+from n_queens_amd import workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_server_without_device_is_an_error():
+    code = ("import n_queens_amd as nq\n"
+            "try:\n    nq.Server()\nexcept nq.NQError as e:\n    print('status', e.status)\n")
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1", PYTHONPATH=ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=120).stdout
+    assert f"status {nq.NQ_ERR_NO_DEVICE}" in out
+
+
+@pytest.mark.gpu
+def test_served_counts_equal_alone_counts():
+    reqs = workloads.poisson_requests(120, lam=1000.0, seed=11)
+    alone = [nq.instance_count(N, cols) for _, N, cols in reqs]
+    with nq.Server() as srv:
+        tickets = [srv.submit(N, cols) for _, N, cols in reqs]
+        got = [srv.wait(t, timeout_ms=60000)[0] for t in tickets]
+        st = srv.stats()
+    assert got == alone
+    assert st["completed"] == len(reqs)
+    for (_, N, cols), c in zip(reqs, got):
+        if N <= 14:
+            assert c == oracle.count_subtree(N, *oracle.encode_prefix(N, cols)), (N, cols)
+
+
+@pytest.mark.gpu
+def test_poisson_arrivals_overlap_and_are_exact():
+    reqs = workloads.poisson_requests(80, lam=400.0, seed=5)
+    alone = [nq.instance_count(N, cols) for _, N, cols in reqs]
+    tickets = [None] * len(reqs)
+    with nq.Server(slots=3) as srv:
+        t0 = time.perf_counter()
+
+        def feed():
+            for i, (at, N, cols) in enumerate(reqs):
+                delay = at - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[i] = srv.submit(N, cols)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        res = [srv.wait(t, timeout_ms=60000) for t in tickets]
+        st = srv.stats()
+    assert [c for c, _ in res] == alone
+    assert all(s["latency_ms"] >= s["queue_ms"] >= 0 for _, s in res)
+    assert st["batches"] >= 2
+
+
+@pytest.mark.gpu
+def test_empty_board_requests(published):
+    with nq.Server() as srv:
+        t = {N: srv.submit(N, []) for N in range(1, 17)}
+        got = {N: srv.wait(x, timeout_ms=60000)[0] for N, x in t.items()}
+    for N in range(4, 17):
+        assert got[N] == published["solutions"][N]
+    assert got[1] == 1 and got[2] == 0 and got[3] == 0
+
+
+@pytest.mark.gpu
+def test_admission_budget_and_invalid_requests():
+    with nq.Server(max_pending_tasks=1000, slot_tasks=5000) as srv:
+        with pytest.raises(nq.NQError) as e:
+            srv.submit(10, [0, 1])
+        assert e.value.status == nq.NQ_ERR_ARG
+        with pytest.raises(nq.NQError) as e:
+            srv.submit(19, [])          # expands to more tasks than one slot holds
+        assert e.value.status == nq.NQ_ERR_BUSY
+        busy = 0
+        tickets = []
+        for _ in range(200):
+            try:
+                tickets.append(srv.submit(15, [0, 2]))
+            except nq.NQError as err:
+                assert err.status == nq.NQ_ERR_BUSY
+                busy += 1
+        counts = {srv.wait(t, timeout_ms=60000)[0] for t in tickets}
+        assert counts == {oracle.count_subtree(15, *oracle.encode_prefix(15, [0, 2]))}
