@@ -30,6 +30,9 @@ METRIC = "N=21 wall s + DFS nodes/s on 1 GPU; batched-mix instances/s vs per-lau
 OPS_PER_NODE = 11          # p=v&-v (2), v^=p (1), c|=p (1), l=(l|p)<<1 (2), r=(r|p)>>1 (2), v'=last&~(c|l|r) (3)
 PEAK_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
 LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
+# HBM bytes per N=21 rows-6 launch of the DFS kernel, from two rocprofv3 --pmc passes scaled by a
+# 256 MiB copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000): profiles/dfs_kernel_n21_rows6_traffic.md.
+TRAFFIC_N21_ROWS6 = (316458 + 139825) * 1024
 
 
 def log(msg: str) -> None:
@@ -175,6 +178,9 @@ def main() -> int:
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--split", type=int, default=-1, help="device split rows (-1: nq_solve's choice)")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--sweep", action="store_true", help="time split 0..2 x block 64/128 and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
@@ -204,14 +210,48 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     d_tasks = torch.from_numpy(tasks.view(np.int32).reshape(-1)).to(dev)
-    d_result = torch.zeros(2, dtype=torch.int64, device=dev)
+    d_result = torch.zeros(3, dtype=torch.int64, device=dev)
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.synchronize()
-    cfg = nq.config_default(N, rows)
+    # The device API's stack_depth bounds the tasks as uploaded (depth `rows`); the library takes
+    # the split rows off it for the DFS after the split.
+    cfg = nq.solve_config(N, rows)
+    cfg.stack_depth = N - 1 - rows
+    if args.split >= 0:
+        cfg = nq.config_default(N, rows)
+        cfg.split = args.split
+    if args.block:
+        cfg.block = args.block
+    cfg_nodes = nq.config_default(N, rows)
+    cfg_nodes.block = cfg.block
 
-    def run(count_nodes: bool) -> None:
-        nq.count_device(N, d_tasks.data_ptr(), ntasks, d_result.data_ptr(), cfg=cfg, count_nodes=count_nodes,
+    def run(count_nodes: bool, c=None) -> None:
+        # The node-counting run counts the unsplit list (nodes at depths rows+1..N-1).
+        nq.count_device(N, d_tasks.data_ptr(), ntasks, d_result.data_ptr(),
+                        cfg=c or (cfg_nodes if count_nodes else cfg), count_nodes=count_nodes,
                         stream=stream.cuda_stream)
+
+    if args.sweep:
+        def timed(c):
+            d_result.zero_()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            a.record(stream)
+            run(False, c)
+            b.record(stream)
+            stream.synchronize()
+            return a.elapsed_time(b), int(d_result.cpu().numpy().view(np.uint64)[0])
+        timed(cfg)
+        for split in (1, 2):
+            for block in (64, 128):
+                for chunk in (1, 4, 16, 64, 256):
+                    c = nq.config_default(N, rows)
+                    c.split, c.block, c.chunk = split, block, chunk
+                    c.stack_depth = N - 1 - rows
+                    ms, s = timed(c)
+                    log(f"sweep N={N} rows={rows} split={split} block={block} chunk={chunk}: {ms:.1f} ms "
+                        f"total={2 * s}")
+        return 0
 
     # Node count (node-counting kernel, once; also the first warmup).
     d_result.zero_()
@@ -220,7 +260,10 @@ def main() -> int:
     stream.synchronize()
     res = d_result.cpu().numpy().view(np.uint64)
     sum0, nodes = int(res[0]), int(res[1])
-    log(f"node-counting run: sum={sum0} total={2*sum0} nodes={nodes}")
+    log(f"node-counting run: sum={sum0} total={2*sum0} nodes={nodes} flags={int(res[2])}")
+    if int(res[2]):
+        log("ERROR: device error flags set")
+        return 1
     for i in range(max(0, args.warmup - 1)):
         d_result.zero_()
         run(False)
@@ -243,7 +286,7 @@ def main() -> int:
     step_ms = [a.elapsed_time(b) for a, b in ev]
     res = d_result.cpu().numpy().view(np.uint64)
     sums = int(res[0])
-    ok = sums == args.steps * sum0
+    ok = sums == args.steps * sum0 and int(res[2]) == 0
     log(f"timed steps ms={['%.1f' % x for x in step_ms]} wall={wall:.3f}s counts_consistent={ok}")
     if not ok:
         log("ERROR: counts differ between steps")
@@ -259,10 +302,12 @@ def main() -> int:
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"N={N} whole-board count, {rows} rows pre-placed (configuration2)", "N": N,
                    "rows": rows, "ntasks": ntasks, "block": cfg.block, "stack_depth": cfg.stack_depth,
-                   "chunk": cfg.chunk, "parallelism": "single GPU, persistent grid"},
+                   "chunk": cfg.chunk, "device_split_rows": cfg.split,
+                   "parallelism": "single GPU, persistent grid"},
         "solutions": 2 * sum0, "nodes": nodes,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tops/s (int32 lane-ops)",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": TRAFFIC_N21_ROWS6 if (N, rows) == (21, 6) else None,
                      "basis": f"{OPS_PER_NODE} int32 ops per DFS node x {nodes} nodes / kernel time; peak = "
                               f"{info['cus']} CUs x 64 lanes x 2.4 GHz"},
     }

@@ -41,6 +41,9 @@ typedef struct nq_config {
     int waves_per_cu; /* resident-wave target that sizes the persistent grid (0 = max)     */
     int chunk;        /* tasks a wave claims per cursor grab (1..64)                      */
     int grid;         /* explicit workgroup count (0 = from waves_per_cu)                  */
+    int split;        /* rows each task is expanded by on the device before the DFS (totals */
+                      /* only; ignored when per-task counts are requested). nq_solve: -1 = */
+                      /* automatic.  Every child keeps the exact count, order is free.      */
 } nq_config;
 
 /* Per-call statistics (all optional). */
@@ -70,6 +73,10 @@ int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t 
  * that check (src/main.cpp:16-19). */
 int nq_config_default(int N, int d0, nq_config *cfg);
 
+/* The configuration nq_solve(N, rows, NULL, ...) uses: defaults for the tasks after the
+ * automatic device split (one more row when 14 or more rows remain below the prefix). */
+int nq_solve_config(int N, int rows, nq_config *cfg);
+
 /* ---- prefix/task generator (host) ------------------------------------------------------
  * Replaces partial_n_queens + partial_n_queens_for_odd (src/n_queens.cpp:74-113) as called by
  * cuda_n_queens (src/n_queens_cuda.cu:91-95): every non-attacking placement of the first
@@ -90,9 +97,11 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
              uint64_t *per_task, uint64_t *nodes, nq_stats *stats);
 
 /* Asynchronous device-pointer form of nq_count, for callers that keep the task list resident
- * in HBM: d_tasks (3*ntasks uint32), d_result (2 uint64: [0] += sum, [1] += nodes; the caller
- * zeroes it) and d_per_task (nullable, ntasks uint64) are device pointers on the current
- * device; stream is a hipStream_t (NULL = default stream).  Enqueues the work and returns;
+ * in HBM: d_tasks (3*ntasks uint32), d_result (3 uint64: [0] += sum, [1] += nodes, [2] |= error
+ * flags, bit 0 = a task needed more stack frames than configured, bit 1 = the device split
+ * overflowed its buffer; the caller zeroes it) and d_per_task (nullable, ntasks uint64) are
+ * device pointers on the current device; stream is a hipStream_t (NULL = default stream).
+ * cfg->stack_depth, when set, bounds N-1-depth over the tasks as given (before any split).  Enqueues the work and returns;
  * the results are valid after the stream synchronises.  Calls on one device share the
  * library's task cursor and are serialised by the library's device lock; each call must be
  * allowed to finish on its stream before a call on another stream of the same device is

@@ -34,7 +34,7 @@ NQ_ERR_STATE = -7
 # Every symbol include/nq.h declares (checked by tests/test_abi.py).
 EXPORTS = (
     "nq_device_count", "nq_last_error", "nq_set_device", "nq_device_info", "nq_config_default",
-    "nq_generate", "nq_count", "nq_count_device", "nq_solve",
+    "nq_generate", "nq_count", "nq_count_device", "nq_solve", "nq_solve_config",
     "nq_batch_plan_create", "nq_batch_plan_launch", "nq_batch_plan_wait", "nq_batch_plan_run",
     "nq_batch_plan_destroy", "nq_batch_count", "nq_instance_count",
     "nq_server_config_default", "nq_server_open", "nq_server_submit", "nq_server_poll", "nq_server_wait",
@@ -53,7 +53,7 @@ class NQError(RuntimeError):
 
 class _Config(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("stack_depth", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
-                ("chunk", ctypes.c_int), ("grid", ctypes.c_int)]
+                ("chunk", ctypes.c_int), ("grid", ctypes.c_int), ("split", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -89,9 +89,10 @@ class Config:
     waves_per_cu: int = 0
     chunk: int = 0
     grid: int = 0
+    split: int = 0
 
     def _c(self) -> _Config:
-        return _Config(self.block, self.stack_depth, self.waves_per_cu, self.chunk, self.grid)
+        return _Config(self.block, self.stack_depth, self.waves_per_cu, self.chunk, self.grid, self.split)
 
 
 _lib = None
@@ -111,6 +112,7 @@ def lib() -> ctypes.CDLL:
     L.nq_device_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                  ctypes.c_char_p, ctypes.c_size_t]
     L.nq_config_default.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Config)]
+    L.nq_solve_config.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Config)]
     L.nq_generate.argtypes = [ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_uint64, _u64p]
     L.nq_count.argtypes = [ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.POINTER(_Config), _u64p, _u64p, _u64p,
                            ctypes.POINTER(_Stats)]
@@ -160,7 +162,14 @@ def device_info(device: int = 0) -> dict:
 def config_default(N: int, d0: int) -> Config:
     c = _Config()
     _check(lib().nq_config_default(N, d0, ctypes.byref(c)))
-    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid)
+    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split)
+
+
+def solve_config(N: int, rows: int) -> Config:
+    """The configuration nq_solve uses for (N, rows), including its device split."""
+    c = _Config()
+    _check(lib().nq_solve_config(N, rows, ctypes.byref(c)))
+    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split)
 
 
 def generate(N: int, rows: int) -> np.ndarray:

@@ -38,6 +38,9 @@ struct Device {
     int wall_khz = 0;  // rate of the kernel's wall_clock64 counter
     uint64_t *trace = nullptr;
     size_t trace_cap = 0;  // in uint64
+    uint32_t *xbuf[2] = {nullptr, nullptr};  // device-side expansion levels (ping-pong)
+    size_t xcap[2] = {0, 0};                 // in tasks
+    unsigned long long *xcount = nullptr;    // [0], [1] task counts of the two buffers, [2] flags
     std::mutex mu;
 };
 
@@ -87,7 +90,8 @@ static int current_device(Device **out) {
         d->clock_khz = prop.clockRate;
         d->name = prop.gcnArchName;
         NQ_HIP(hipMalloc(&d->state, sizeof(DevState)));
-        NQ_HIP(hipMalloc(&d->result, 2 * sizeof(uint64_t)));
+        NQ_HIP(hipMalloc(&d->result, 3 * sizeof(uint64_t)));
+        NQ_HIP(hipMalloc(&d->xcount, 3 * sizeof(unsigned long long)));
         NQ_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
         NQ_HIP(hipEventCreate(&d->ev0));
         NQ_HIP(hipEventCreate(&d->ev1));
@@ -120,6 +124,7 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
         if (in->waves_per_cu) c.waves_per_cu = in->waves_per_cu;
         if (in->chunk) c.chunk = in->chunk;
         if (in->grid) c.grid = in->grid;
+        c.split = in->split;
     }
     int rc = check_cfg_shape(c);
     if (rc) return rc;
@@ -195,6 +200,50 @@ static void trace_stats(const std::vector<uint64_t> &tr, int wall_khz, double *b
     *tail_ms = static_cast<double>(t1 - first_end) / wall_khz;
 }
 
+static int ensure_xbuf(Device *d, int i, uint64_t n) {
+    if (d->xcap[i] >= n && d->xbuf[i]) return NQ_OK;
+    if (d->xbuf[i]) (void)hipFree(d->xbuf[i]);
+    d->xbuf[i] = nullptr;
+    d->xcap[i] = 0;
+    NQ_HIP(hipMalloc(&d->xbuf[i], std::max<uint64_t>(n, 1) * 3 * sizeof(uint32_t)));
+    d->xcap[i] = n;
+    return NQ_OK;
+}
+
+// Device expansion budget: the levels stop before a buffer would exceed this many tasks.
+static constexpr uint64_t kMaxExpandedTasks = 2ull << 30;  // 24 GiB of 12-B tasks
+
+// Enqueue up to `split` one-row expansions of (tasks, n) on the device.  A task at depth d has at
+// most max(1, N-d) outputs (a full board is carried over), which bounds each level's buffer.  On return *tasks_out / *n_dev / *bound
+// describe the list the DFS consumes and *levels how many rows were expanded.
+static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, uint64_t n, int split, hipStream_t s,
+                         const uint32_t **tasks_out, const unsigned long long **n_dev, uint64_t *bound, int *levels) {
+    if (split > 0) {
+        hipError_t e = launch_zero(d->xcount + 2, 1, s);
+        if (e != hipSuccess) return fail_hip(e, "launch_zero");
+    }
+    const uint32_t *in = tasks;
+    const unsigned long long *in_dev = nullptr;
+    uint64_t b = n;
+    int j = 0;
+    for (; j < split && d0 + j + 2 < N; ++j) {
+        const uint64_t nb = b * static_cast<uint64_t>(std::max(1, N - d0 - j));
+        if (nb > kMaxExpandedTasks) break;
+        int rc = ensure_xbuf(d, j & 1, nb);
+        if (rc) return rc;
+        hipError_t e = launch_expand(in, b, in_dev, N, d->xbuf[j & 1], nb, &d->xcount[j & 1], &d->xcount[2], s);
+        if (e != hipSuccess) return fail_hip(e, "launch_expand");
+        in = d->xbuf[j & 1];
+        in_dev = &d->xcount[j & 1];
+        b = nb;
+    }
+    *tasks_out = in;
+    *n_dev = in_dev;
+    *bound = b;
+    *levels = j;
+    return NQ_OK;
+}
+
 static int min_depth(int N, const uint32_t *tasks, uint64_t ntasks) {
     int d0 = N;
     for (uint64_t i = 0; i < ntasks; ++i) d0 = std::min(d0, __builtin_popcount(tasks[3 * i]));
@@ -244,6 +293,7 @@ int nq_config_default(int N, int d0, nq_config *cfg) {
     cfg->waves_per_cu = 0;
     cfg->chunk = 1;
     cfg->grid = 0;
+    cfg->split = 0;
     return NQ_OK;
 }
 
@@ -270,14 +320,28 @@ int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_co
     // The task depths are not visible from the host here: size the stack for the shallowest
     // prefix the caller allows (cfg->stack_depth), else for depth 1; the kernel flags and skips
     // any task that would need more.
-    nq_config c;
     const int d0 = (cfg && cfg->stack_depth) ? std::max(0, N - 1 - cfg->stack_depth) : 1;
-    rc = resolve_config(d, N, d0, ntasks, cfg, count_nodes != 0, d_per_task != nullptr, &c);
-    if (rc) return rc;
+    const int split = (cfg && !d_per_task && !count_nodes) ? std::max(0, cfg->split) : 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> g(d->mu);
-    LaunchArgs a{d_tasks, ntasks, N, c.block, c.grid, c.stack_depth, c.chunk, d->state, d_per_task, d_result,
+    const uint32_t *t = d_tasks;
+    const unsigned long long *n_dev = nullptr;
+    uint64_t bound = ntasks;
+    int levels = 0;
+    if (split) {
+        rc = enqueue_split(d, N, d0, d_tasks, ntasks, split, s, &t, &n_dev, &bound, &levels);
+        if (rc) return rc;
+    }
+    nq_config in = cfg ? *cfg : nq_config{};
+    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - levels);
+    nq_config c;
+    rc = resolve_config(d, N, d0 + levels, bound, &in, count_nodes != 0, d_per_task != nullptr, &c);
+    if (rc) return rc;
+    LaunchArgs a{t, bound, N, c.block, c.grid, c.stack_depth, c.chunk, d->state, d_per_task, d_result,
                  count_nodes != 0};
-    hipError_t e = launch_count(a, static_cast<hipStream_t>(stream));
+    a.ntasks_dev = n_dev;
+    a.xflags = levels ? d->xcount + 2 : nullptr;
+    hipError_t e = launch_count(a, s);
     if (e != hipSuccess) return fail_hip(e, "launch_count");
     return NQ_OK;
 }
@@ -292,9 +356,7 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     int rc = current_device(&d);
     if (rc) return rc;
     const int d0 = min_depth(N, tasks, ntasks);
-    nq_config c;
-    rc = resolve_config(d, N, d0, ntasks, cfg, nodes != nullptr, per_task != nullptr, &c);
-    if (rc) return rc;
+    const int split = (cfg && !per_task && !nodes) ? std::max(0, cfg->split) : 0;
 
     std::lock_guard<std::mutex> g(d->mu);
     rc = ensure_tasks(d, ntasks);
@@ -303,24 +365,39 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
         rc = ensure_per_task(d, ntasks);
         if (rc) return rc;
     }
+    const double t1 = now_ms();
+    if (ntasks)
+        NQ_HIP(hipMemcpyAsync(d->tasks, tasks, ntasks * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream));
+    NQ_HIP(hipMemsetAsync(d->result, 0, 3 * sizeof(uint64_t), d->stream));
+    NQ_HIP(hipStreamSynchronize(d->stream));
+    const double t2 = now_ms();
+    NQ_HIP(hipEventRecord(d->ev0, d->stream));
+    const uint32_t *t = d->tasks;
+    const unsigned long long *n_dev = nullptr;
+    uint64_t bound = ntasks;
+    int levels = 0;
+    if (split) {
+        rc = enqueue_split(d, N, d0, d->tasks, ntasks, split, d->stream, &t, &n_dev, &bound, &levels);
+        if (rc) return rc;
+    }
+    nq_config in = cfg ? *cfg : nq_config{};
+    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - levels);
+    nq_config c;
+    rc = resolve_config(d, N, std::min(N, d0 + levels), bound, &in, nodes != nullptr, per_task != nullptr, &c);
+    if (rc) return rc;
     const size_t nwaves = static_cast<size_t>(c.grid) * (c.block / 64);
     if (stats) {
         rc = ensure_trace(d, 2 * nwaves);
         if (rc) return rc;
     }
-    const double t1 = now_ms();
-    if (ntasks)
-        NQ_HIP(hipMemcpyAsync(d->tasks, tasks, ntasks * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream));
-    NQ_HIP(hipMemsetAsync(d->result, 0, 2 * sizeof(uint64_t), d->stream));
-    NQ_HIP(hipStreamSynchronize(d->stream));
-    const double t2 = now_ms();
-    LaunchArgs a{d->tasks, ntasks, N, c.block, c.grid, c.stack_depth, c.chunk, d->state,
+    LaunchArgs a{t, bound, N, c.block, c.grid, c.stack_depth, c.chunk, d->state,
                  per_task ? d->per_task : nullptr, d->result, nodes != nullptr, stats ? d->trace : nullptr};
-    NQ_HIP(hipEventRecord(d->ev0, d->stream));
+    a.ntasks_dev = n_dev;
+    a.xflags = levels ? d->xcount + 2 : nullptr;
     hipError_t e = launch_count(a, d->stream);
     if (e != hipSuccess) return fail_hip(e, "launch_count");
     NQ_HIP(hipEventRecord(d->ev1, d->stream));
-    uint64_t res[2] = {0, 0};
+    uint64_t res[3] = {0, 0, 0};
     DevState st;
     NQ_HIP(hipMemcpyAsync(res, d->result, sizeof res, hipMemcpyDeviceToHost, d->stream));
     NQ_HIP(hipMemcpyAsync(&st, d->state, sizeof st, hipMemcpyDeviceToHost, d->stream));
@@ -332,7 +409,8 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
         NQ_HIP(hipMemcpyAsync(tr.data(), d->trace, tr.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, d->stream));
     }
     NQ_HIP(hipStreamSynchronize(d->stream));
-    if (st.error) return fail(NQ_ERR_DEPTH, "a task needed more stack frames than configured");
+    if (res[2] & 2) return fail(NQ_ERR_STATE, "device expansion overflowed its buffer");
+    if (st.error || (res[2] & 1)) return fail(NQ_ERR_DEPTH, "a task needed more stack frames than configured");
     *sum = res[0];
     if (nodes) *nodes = res[1];
     if (stats) {
@@ -350,6 +428,23 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     return NQ_OK;
 }
 
+// Rows nq_solve expands on the device: deep whole-board tasks get finer (shorter drain tail) and
+// need fewer LDS frames (more waves per CU).  Measured on N=20/21 with 6 rows pre-placed:
+// 14 free rows -> 1 row (1.45 -> 1.26 s), 15 free rows -> 2 rows (13.2 -> 10.4 s).
+static int auto_split(int N, int rows) {
+    const int free_rows = N - rows;
+    return free_rows >= 15 ? 2 : (free_rows >= 14 ? 1 : 0);
+}
+
+int nq_solve_config(int N, int rows, nq_config *cfg) {
+    if (!cfg) return fail(NQ_ERR_ARG, "cfg is NULL");
+    if (N < 1 || N > MAX_N || rows < 1 || rows > N) return fail(NQ_ERR_ARG, "N must be 1..27 and rows 1..N");
+    const int split = auto_split(N, rows);
+    nq_config_default(N, std::min(N, rows + split), cfg);
+    cfg->split = split;
+    return NQ_OK;
+}
+
 int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats) {
     const double t0 = now_ms();
     if (!total) return fail(NQ_ERR_ARG, "total is NULL");
@@ -360,7 +455,9 @@ int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *s
     const double t1 = now_ms();
     const uint64_t n = tasks.size() / 3;
     uint64_t sum = 0;
-    int rc = nq_count(N, tasks.data(), n, cfg, &sum, nullptr, nullptr, stats);
+    nq_config c = cfg ? *cfg : nq_config{};
+    if (!cfg || c.split < 0) c.split = auto_split(N, rows);
+    int rc = nq_count(N, tasks.data(), n, &c, &sum, nullptr, nullptr, stats);
     if (rc) return rc;
     *total = 2 * sum;  // symmetry fold, src/n_queens_cuda.cu:181-183
     if (stats) {

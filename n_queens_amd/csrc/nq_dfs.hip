@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "nq_kernels.h"
 
 namespace nq {
@@ -49,9 +51,16 @@ __device__ __forceinline__ uint64_t first_lane_u64(uint64_t x) {
 
 // Claim one task index for every active lane of the wave.  wq[0..1] is the wave's claimed but
 // unused range [next, end) in LDS (kept in memory, not registers, because lanes leave the
-// claiming loop at different times and must all see the same range afterwards).
-__device__ __forceinline__ uint64_t claim(unsigned long long *wq, unsigned long long *cursor,
+// claiming loop at different times and must all see the same range afterwards).  Lane 0 of the
+// claiming lanes writes it and every lane reads it, so the accesses are volatile: the compiler
+// must not keep a lane's earlier read in a register (that handed out task indices twice).  LDS
+// operations of one wave complete in order, so the next claim sees the last write.
+__device__ __forceinline__ uint64_t claim(unsigned long long *wq_, unsigned long long *cursor,
                                           uint32_t chunk) {
+    // The explicit LDS address space keeps these ds_read/ds_write (a plain volatile pointer
+    // would become flat accesses).
+    volatile __attribute__((address_space(3))) unsigned long long *wq =
+        (volatile __attribute__((address_space(3))) unsigned long long *)wq_;
     const uint64_t mask = __ballot(1);
     const uint32_t k = __popcll(mask);
     const uint32_t rank = lane_rank(mask);
@@ -155,7 +164,9 @@ template <bool NODES, bool PER_TASK>
 __global__ void __launch_bounds__(MAX_BLOCK)
 dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint32_t chunk,
            uint32_t stack_depth, DevState *__restrict__ st, uint64_t *__restrict__ per_task,
-           uint64_t *__restrict__ trace) {
+           uint64_t *__restrict__ trace, const unsigned long long *__restrict__ ntasks_dev) {
+    // Task count produced on the device (expansion), never beyond the buffer bound `ntasks`.
+    if (ntasks_dev) ntasks = min(ntasks, static_cast<uint64_t>(*ntasks_dev));
     extern __shared__ uint4 stack[];  // [stack_depth][blockDim.x]
     __shared__ unsigned long long wq[MAX_WAVES_PER_BLOCK][2];
     __shared__ unsigned long long red[MAX_WAVES_PER_BLOCK][2];
@@ -365,10 +376,67 @@ __global__ void reset_state_kernel(DevState *st) {
     }
 }
 
-__global__ void fold_result_kernel(const DevState *st, uint64_t *result) {
+__global__ void fold_result_kernel(const DevState *st, uint64_t *result, const unsigned long long *xflags) {
     if (threadIdx.x == 0) {
         result[0] += st->total;
         result[1] += st->nodes;
+        result[2] |= st->error | (xflags ? *xflags : 0ull);
+    }
+}
+
+// Expand every task by one row: out gets (cur|p, (left|p)<<1, (right|p)>>1) for every legal
+// queen p of the next row, and a full board is carried over as it is; the counts of a task's
+// outputs sum to its own count.  Outputs are placed through one wave-aggregated atomic on
+// *out_n, so their order is not the input's: only totals use this.
+__global__ void expand_kernel(const uint32_t *__restrict__ in, uint64_t n, const unsigned long long *__restrict__ n_dev,
+                              uint32_t N, uint32_t *__restrict__ out, uint64_t cap,
+                              unsigned long long *__restrict__ out_n, unsigned long long *__restrict__ flags) {
+    if (n_dev) n = min(n, static_cast<uint64_t>(*n_dev));
+    const uint32_t last = (1u << N) - 1u;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t c = 0, l = 0, r = 0, v = 0;
+        if (i < n) {
+            c = in[3 * i];
+            l = in[3 * i + 1];
+            r = in[3 * i + 2];
+            v = last & ~(c | l | r);
+        }
+        // A full board has no children but counts 1: carry it over unchanged.
+        const bool full = (i < n) && c == last;
+        // Wave-inclusive prefix sum of the child counts, one atomic per wave.
+        const uint32_t k = full ? 1u : __popc(v);
+        uint32_t incl = k;
+        const uint32_t lane = threadIdx.x & 63u;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t wave_total = __shfl(incl, 63, 64);
+        unsigned long long wbase = 0;
+        if (lane == 63 && wave_total) wbase = atomicAdd(out_n, static_cast<unsigned long long>(wave_total));
+        wbase = __shfl(wbase, 63, 64);
+        uint64_t o = wbase + (incl - k);
+        if (o + k > cap) {
+            // More outputs than the buffer holds (the caller's depth bound was wrong): write
+            // nothing past it and report; the consumers clamp their counts to `cap`.
+            if (k) atomicOr(flags, 2ull);
+            continue;
+        }
+        if (full) {
+            out[3 * o] = c;
+            out[3 * o + 1] = l;
+            out[3 * o + 2] = r;
+        }
+        while (v) {
+            const uint32_t p = v & (0u - v);
+            v ^= p;
+            out[3 * o] = c | p;
+            out[3 * o + 1] = (l | p) << 1;
+            out[3 * o + 2] = (r | p) >> 1;
+            ++o;
+        }
     }
 }
 
@@ -427,6 +495,22 @@ hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, block, lds);
 }
 
+hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t stream) {
+    zero_u64_kernel<<<static_cast<unsigned>(std::min<uint64_t>(64, (n + 255) / 256 + 1)), 256, 0, stream>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand(const uint32_t *in, uint64_t n_max, const unsigned long long *n_dev, int N, uint32_t *out,
+                         uint64_t cap, unsigned long long *out_n, unsigned long long *flags, hipStream_t stream) {
+    zero_u64_kernel<<<1, 64, 0, stream>>>(out_n, 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint64_t blocks = std::min<uint64_t>((n_max + 255) / 256, 16384);
+    expand_kernel<<<static_cast<unsigned>(std::max<uint64_t>(1, blocks)), 256, 0, stream>>>(
+        in, n_max, n_dev, static_cast<uint32_t>(N), out, cap, out_n, flags);
+    return hipGetLastError();
+}
+
 hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
     reset_state_kernel<<<1, 64, 0, stream>>>(a.state);
     hipError_t e = hipGetLastError();
@@ -440,22 +524,22 @@ hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
     if (a.count_nodes) {
         if (a.per_task)
             dfs_kernel<true, true><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                      a.per_task, a.trace);
+                                                                      a.per_task, a.trace, a.ntasks_dev);
         else
             dfs_kernel<true, false><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                       a.per_task, a.trace);
+                                                                       a.per_task, a.trace, a.ntasks_dev);
     } else {
         if (a.per_task)
             dfs_kernel<false, true><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                       a.per_task, a.trace);
+                                                                       a.per_task, a.trace, a.ntasks_dev);
         else
             dfs_kernel<false, false><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                        a.per_task, a.trace);
+                                                                        a.per_task, a.trace, a.ntasks_dev);
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (a.result) {
-        fold_result_kernel<<<1, 64, 0, stream>>>(a.state, a.result);
+        fold_result_kernel<<<1, 64, 0, stream>>>(a.state, a.result, a.xflags);
         e = hipGetLastError();
     }
     return e;

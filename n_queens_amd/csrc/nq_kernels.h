@@ -32,9 +32,11 @@ struct LaunchArgs {
     int chunk;        // tasks per cursor grab
     DevState *state;
     uint64_t *per_task;   // nullable
-    uint64_t *result;     // nullable: [0] += total, [1] += nodes at the end of the launch
+    uint64_t *result;     // nullable: [0] += total, [1] += nodes, [2] |= error flags at the end
     bool count_nodes;
     uint64_t *trace = nullptr;  // nullable: per wave {start, end} wall-clock ticks (grid * block/64 * 2)
+    const unsigned long long *ntasks_dev = nullptr;  // nullable: task count on the device (<= ntasks)
+    const unsigned long long *xflags = nullptr;      // nullable: expansion flags folded into result[2]
 };
 
 struct BatchArgs {
@@ -52,6 +54,15 @@ hipError_t batch_occupancy(int block, int stack_depth, int *blocks_per_cu);
 
 // Enqueue: reset state, DFS, (optional) fold into result.  Returns a hipError_t.
 hipError_t launch_count(const LaunchArgs &a, hipStream_t stream);
+
+// Expand tasks (3 words each) by one row into out (children in arbitrary order); *out_n is
+// zeroed and receives the child count.  The input count is n_max, or min(n_max, *n_dev) when
+// n_dev is set.  Nothing is written past cap tasks: overflow sets bit 1 of *flags.
+hipError_t launch_expand(const uint32_t *in, uint64_t n_max, const unsigned long long *n_dev, int N, uint32_t *out,
+                         uint64_t cap, unsigned long long *out_n, unsigned long long *flags, hipStream_t stream);
+
+// Zero n 64-bit words with a kernel (graph-capturable, unlike a memset node).
+hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t stream);
 
 // Resident workgroups per CU for the DFS kernel at (block, stack_depth).
 hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task, int *blocks_per_cu);

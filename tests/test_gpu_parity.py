@@ -121,3 +121,66 @@ def test_cli_end_to_end():
     assert p.returncode == 0
     assert "Use " in p.stdout and "to generate 15550 subproblems!" in p.stdout
     assert "cuda 13 queens result 73712, calc time:" in p.stdout
+
+
+@pytest.mark.parametrize("N,rows", [(10, 2), (13, 3), (16, 4), (17, 5)])
+def test_device_split_keeps_totals(N, rows, published):
+    """Tasks expanded on the device by 1..3 rows give the same totals."""
+    t = nq.generate(N, rows)
+    base = nq.count(N, t).sum
+    assert 2 * base == published["solutions"][N]
+    for split in (1, 2, 3):
+        r = nq.count(N, t, cfg=nq.Config(split=split))
+        assert r.sum == base, (N, rows, split)
+        total, _ = nq.solve(N, rows, nq.Config(split=split))
+        assert total == published["solutions"][N]
+
+
+def test_device_split_mixed_depth_list():
+    """Expansion of a list holding full boards, depth N-1 and dead prefixes keeps every count."""
+    N = 8
+    rows = []
+    for cols in ([0, 4, 7, 5, 2, 6, 1, 3], [0, 4, 7, 5, 2, 6, 1], [0, 2, 4], [1, 3, 5, 7], [2, 4, 1, 7, 0], [], [3]):
+        rows.append(oracle.encode_prefix(N, cols))
+    t = np.array(rows, dtype=np.uint32)
+    expect = oracle.count_tasks(N, t)
+    for split in (0, 1, 2, 4):
+        assert nq.count(N, t, cfg=nq.Config(split=split)).sum == expect, split
+
+
+@pytest.mark.parametrize("chunk", [2, 8, 64, 1000])
+@pytest.mark.parametrize("split", [0, 1, 2])
+def test_chunked_claims_with_device_split(chunk, split, published):
+    """Chunked cursor claims stay exact when many claimed tasks die at once (after a device
+    split most expanded tasks are dead, so lanes retry the wave's LDS range many times)."""
+    for N, rows, block in ((17, 5, 64), (15, 3, 256), (12, 2, 128)):
+        t = nq.generate(N, rows)
+        r = nq.count(N, t, cfg=nq.Config(block=block, chunk=chunk, split=split))
+        assert 2 * r.sum == published["solutions"][N], (N, rows, block, chunk, split)
+
+
+@pytest.mark.parametrize("chunk", [3, 64])
+def test_chunked_claims_per_task(chunk, ref_vectors):
+    """Per-task counts (no split) with chunked claims: every task is counted exactly once."""
+    N, rows = 13, 5
+    t = nq.generate(N, rows)
+    r = nq.count(N, t, per_task=True, cfg=nq.Config(block=64, chunk=chunk))
+    assert np.array_equal(r.per_task, oracle.count_tasks(N, t, per_task=True)[1])
+
+
+@pytest.mark.parametrize("chunk", [8, 64])
+def test_n20_split2_chunked(chunk, published):
+    total, _ = nq.solve(20, 6, nq.Config(block=64, chunk=chunk, split=2))
+    assert total == published["solutions"][20]
+
+
+@pytest.mark.parametrize("chunk", [4, 64])
+def test_batch_chunked_claims(chunk):
+    from n_queens_amd import workloads
+    mix = workloads.batch_mix(96, seed=5)
+    a, _ = nq.batch_count(mix, nq.Config(chunk=1))
+    b, _ = nq.batch_count(mix, nq.Config(chunk=chunk))
+    assert np.array_equal(a, b)
+    for (N, cols), got in zip(mix, b):
+        if N <= 15:
+            assert got == oracle.count_subtree(N, *oracle.encode_prefix(N, cols)), (N, cols)
