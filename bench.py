@@ -30,9 +30,10 @@ METRIC = "N=21 wall s + DFS nodes/s on 1 GPU; batched-mix instances/s vs per-lau
 OPS_PER_NODE = 11          # p=v&-v (2), v^=p (1), c|=p (1), l=(l|p)<<1 (2), r=(r|p)>>1 (2), v'=last&~(c|l|r) (3)
 PEAK_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
 LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
-# HBM bytes per N=21 rows-6 launch of the DFS kernel, from two rocprofv3 --pmc passes scaled by a
-# 256 MiB copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000): profiles/dfs_kernel_n21_rows6_traffic.md.
-TRAFFIC_N21_ROWS6 = (316458 + 139825) * 1024
+# HBM bytes per N=21 rows-6 launch of the DFS kernel (3 rows split on the device), from two
+# rocprofv3 --pmc passes scaled by a 256 MiB copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000):
+# profiles/dfs_kernel_n21_rows6_traffic.md.
+TRAFFIC_N21_ROWS6 = (25399474 + 3278870) * 1024
 
 
 def log(msg: str) -> None:
@@ -180,7 +181,6 @@ def main() -> int:
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--split", type=int, default=-1, help="device split rows (-1: nq_solve's choice)")
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--sweep", action="store_true", help="time split 0..2 x block 64/128 and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
@@ -230,28 +230,6 @@ def main() -> int:
         nq.count_device(N, d_tasks.data_ptr(), ntasks, d_result.data_ptr(),
                         cfg=c or (cfg_nodes if count_nodes else cfg), count_nodes=count_nodes,
                         stream=stream.cuda_stream)
-
-    if args.sweep:
-        def timed(c):
-            d_result.zero_()
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            torch.cuda.synchronize()
-            a.record(stream)
-            run(False, c)
-            b.record(stream)
-            stream.synchronize()
-            return a.elapsed_time(b), int(d_result.cpu().numpy().view(np.uint64)[0])
-        timed(cfg)
-        for split in (1, 2):
-            for block in (64, 128):
-                for chunk in (1, 4, 16, 64, 256):
-                    c = nq.config_default(N, rows)
-                    c.split, c.block, c.chunk = split, block, chunk
-                    c.stack_depth = N - 1 - rows
-                    ms, s = timed(c)
-                    log(f"sweep N={N} rows={rows} split={split} block={block} chunk={chunk}: {ms:.1f} ms "
-                        f"total={2 * s}")
-        return 0
 
     # Node count (node-counting kernel, once; also the first warmup).
     d_result.zero_()

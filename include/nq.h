@@ -39,12 +39,19 @@ typedef struct nq_config {
     int block;        /* threads per workgroup, a multiple of 64, <= 256                  */
     int stack_depth;  /* LDS frames per lane; must be >= N - 1 - (prefix depth)           */
     int waves_per_cu; /* resident-wave target that sizes the persistent grid (0 = max)     */
-    int chunk;        /* tasks a wave claims per cursor grab (1..64)                      */
+    int chunk;        /* tasks a wave claims per cursor grab (1..4096)                    */
     int grid;         /* explicit workgroup count (0 = from waves_per_cu)                  */
     int split;        /* rows each task is expanded by on the device before the DFS (totals */
                       /* only; ignored when per-task counts are requested). nq_solve: -1 = */
                       /* automatic.  Every child keeps the exact count, order is free.      */
+    int push;         /* DFS push form: NQ_PUSH_AUTO (0) takes NQ_PUSH_ALWAYS when its spare */
+                      /* stack level costs no resident waves, else NQ_PUSH_MASKED.  Both    */
+                      /* give identical counts.                                              */
 } nq_config;
+
+#define NQ_PUSH_AUTO 0
+#define NQ_PUSH_MASKED 1   /* exec-masked write of frames that keep candidates               */
+#define NQ_PUSH_ALWAYS 2   /* unconditional write, one spare LDS level per lane              */
 
 /* Per-call statistics (all optional). */
 typedef struct nq_stats {
@@ -68,7 +75,7 @@ int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t 
 
 /* Fill cfg with defaults for (N, shallowest prefix depth d0): 128-thread workgroups, the
  * smallest legal stack depth N-1-d0 (LDS per lane decides occupancy, so no frame is wasted),
- * chunk 1, and a persistent grid of every workgroup that fits on the device.  The reference's
+ * chunk 16, push NQ_PUSH_AUTO, and a persistent grid of every workgroup that fits on the device.  The reference's
  * compile-time presets only decide which (N, rows) pairs its CLI accepts; the CLI here keeps
  * that check (src/main.cpp:16-19). */
 int nq_config_default(int N, int d0, nq_config *cfg);
@@ -102,7 +109,8 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
  * overflowed its buffer; the caller zeroes it) and d_per_task (nullable, ntasks uint64) are
  * device pointers on the current device; stream is a hipStream_t (NULL = default stream).
  * cfg->stack_depth, when set, bounds N-1-depth over the tasks as given (before any split).  Enqueues the work and returns;
- * the results are valid after the stream synchronises.  Calls on one device share the
+ * the results are valid after the stream synchronises.  With cfg->split > 0 the call first
+ * synchronizes the stream once per split row, to size that row's expansion buffer exactly.  Calls on one device share the
  * library's task cursor and are serialised by the library's device lock; each call must be
  * allowed to finish on its stream before a call on another stream of the same device is
  * issued. */
@@ -116,7 +124,7 @@ int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_co
 int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats);
 
 /* ---- batched instances (no reference counterpart) -------------------------------------
- * An instance is a board of size N (1..27) whose first k rows (0 <= k <= N) hold queens at
+ * An instance is a board of size N (1..28) whose first k rows (0 <= k <= N) hold queens at
  * columns cols[0..k-1]; its count is the number of completions of that prefix — what the
  * reference's n_queens(N, cur, left, right, sum) (src/n_queens.cpp:8-21) returns for the
  * prefix masks — with no symmetry fold.  k = 0 is the empty board.
@@ -142,7 +150,7 @@ typedef struct nq_batch_stats {
 typedef struct nq_batch_plan nq_batch_plan;
 
 /* Returns NQ_ERR_ARG (message names the instance) for N out of range, k > N, a column out of
- * range or an attacking prefix.  cfg: block, chunk, waves_per_cu, grid as for nq_count;
+ * range or an attacking prefix.  cfg: block, chunk, waves_per_cu, grid, push as for nq_count;
  * stack_depth (0 = 12 levels per task) bounds the per-task depth. */
 int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, nq_batch_plan **plan);
 int nq_batch_plan_launch(nq_batch_plan *plan);   /* enqueue one replay, asynchronous         */

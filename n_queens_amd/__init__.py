@@ -30,6 +30,7 @@ NQ_ERR_DEPTH = -4
 NQ_ERR_NOMEM = -5
 NQ_ERR_BUSY = -6
 NQ_ERR_STATE = -7
+NQ_PUSH_AUTO, NQ_PUSH_MASKED, NQ_PUSH_ALWAYS = 0, 1, 2
 
 # Every symbol include/nq.h declares (checked by tests/test_abi.py).
 EXPORTS = (
@@ -53,7 +54,7 @@ class NQError(RuntimeError):
 
 class _Config(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("stack_depth", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
-                ("chunk", ctypes.c_int), ("grid", ctypes.c_int), ("split", ctypes.c_int)]
+                ("chunk", ctypes.c_int), ("grid", ctypes.c_int), ("split", ctypes.c_int), ("push", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -90,9 +91,11 @@ class Config:
     chunk: int = 0
     grid: int = 0
     split: int = 0
+    push: int = 0
 
     def _c(self) -> _Config:
-        return _Config(self.block, self.stack_depth, self.waves_per_cu, self.chunk, self.grid, self.split)
+        return _Config(self.block, self.stack_depth, self.waves_per_cu, self.chunk, self.grid, self.split,
+                       self.push)
 
 
 _lib = None
@@ -162,14 +165,14 @@ def device_info(device: int = 0) -> dict:
 def config_default(N: int, d0: int) -> Config:
     c = _Config()
     _check(lib().nq_config_default(N, d0, ctypes.byref(c)))
-    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split)
+    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push)
 
 
 def solve_config(N: int, rows: int) -> Config:
     """The configuration nq_solve uses for (N, rows), including its device split."""
     c = _Config()
     _check(lib().nq_solve_config(N, rows, ctypes.byref(c)))
-    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split)
+    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push)
 
 
 def generate(N: int, rows: int) -> np.ndarray:

@@ -40,7 +40,8 @@ struct Device {
     size_t trace_cap = 0;  // in uint64
     uint32_t *xbuf[2] = {nullptr, nullptr};  // device-side expansion levels (ping-pong)
     size_t xcap[2] = {0, 0};                 // in tasks
-    unsigned long long *xcount = nullptr;    // [0], [1] task counts of the two buffers, [2] flags
+    unsigned long long *xcount = nullptr;    // [0], [1] task counts of the two buffers, [2] flags,
+                                             // [3] child count of the next level (exact sizing)
     std::mutex mu;
 };
 
@@ -91,7 +92,7 @@ static int current_device(Device **out) {
         d->name = prop.gcnArchName;
         NQ_HIP(hipMalloc(&d->state, sizeof(DevState)));
         NQ_HIP(hipMalloc(&d->result, 3 * sizeof(uint64_t)));
-        NQ_HIP(hipMalloc(&d->xcount, 3 * sizeof(unsigned long long)));
+        NQ_HIP(hipMalloc(&d->xcount, 4 * sizeof(unsigned long long)));
         NQ_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
         NQ_HIP(hipEventCreate(&d->ev0));
         NQ_HIP(hipEventCreate(&d->ev1));
@@ -107,6 +108,7 @@ static int check_cfg_shape(const nq_config &c) {
     if (c.chunk < 1 || c.chunk > 4096) return fail(NQ_ERR_ARG, "chunk must be 1..4096");
     if (c.stack_depth < 1 || c.stack_depth > 32) return fail(NQ_ERR_ARG, "stack_depth must be 1..32");
     if (c.waves_per_cu < 0 || c.grid < 0) return fail(NQ_ERR_ARG, "negative waves_per_cu/grid");
+    if (c.push < 0 || c.push > NQ_PUSH_ALWAYS) return fail(NQ_ERR_ARG, "push must be 0, 1 or 2");
     return NQ_OK;
 }
 
@@ -125,6 +127,7 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
         if (in->chunk) c.chunk = in->chunk;
         if (in->grid) c.grid = in->grid;
         c.split = in->split;
+        c.push = in->push;
     }
     int rc = check_cfg_shape(c);
     if (rc) return rc;
@@ -134,13 +137,17 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
                  N, d0, frames_needed(N, d0));
         return fail(NQ_ERR_DEPTH, buf);
     }
-    if (c.grid == 0) {
-        int bpc = 0;
-        NQ_HIP(dfs_occupancy(c.block, c.stack_depth, nodes, per_task, &bpc));
-        if (bpc <= 0) return fail(NQ_ERR_ARG, "configuration does not fit on a CU (LDS)");
-        if (c.waves_per_cu > 0) bpc = std::max(1, std::min(bpc, c.waves_per_cu / (c.block / 64)));
-        c.grid = d->cus * bpc;
-    }
+    // Resident workgroups per CU for both push forms (capped by the waves_per_cu target); the
+    // unconditional push is taken when its spare stack level costs no workgroup.
+    int occ[2] = {0, 0};
+    NQ_HIP(dfs_occupancy(c.block, c.stack_depth, nodes, per_task, false, &occ[0]));
+    NQ_HIP(dfs_occupancy(c.block, c.stack_depth, nodes, per_task, true, &occ[1]));
+    if (c.waves_per_cu > 0)
+        for (int &o : occ) o = std::min(o, std::max(1, c.waves_per_cu / (c.block / 64)));
+    if (c.push == 0) c.push = (occ[1] > 0 && occ[1] >= occ[0]) ? NQ_PUSH_ALWAYS : NQ_PUSH_MASKED;
+    const int bpc = occ[c.push == NQ_PUSH_ALWAYS ? 1 : 0];
+    if (bpc <= 0) return fail(NQ_ERR_ARG, "configuration does not fit on a CU (LDS)");
+    if (c.grid == 0) c.grid = d->cus * bpc;
     const uint64_t lanes_needed = (ntasks + c.block - 1) / c.block;
     if (static_cast<uint64_t>(c.grid) > lanes_needed) c.grid = static_cast<int>(std::max<uint64_t>(1, lanes_needed));
     *out = c;
@@ -213,11 +220,15 @@ static int ensure_xbuf(Device *d, int i, uint64_t n) {
 // Device expansion budget: the levels stop before a buffer would exceed this many tasks.
 static constexpr uint64_t kMaxExpandedTasks = 2ull << 30;  // 24 GiB of 12-B tasks
 
-// Enqueue up to `split` one-row expansions of (tasks, n) on the device.  A task at depth d has at
-// most max(1, N-d) outputs (a full board is carried over), which bounds each level's buffer.  On return *tasks_out / *n_dev / *bound
-// describe the list the DFS consumes and *levels how many rows were expanded.
+// Enqueue up to `split` one-row expansions of (tasks, n) on the device (totals only; a full
+// board is carried over).  The child count of every level is reduced on the device and read
+// back first (one stream synchronization per row) so that each level's buffer has its exact
+// size (the worst case, every task having N - depth children, is ~10x the real count per row);
+// a level larger than kMaxExpandedTasks ends the split there.  On return *tasks_out / *n_dev /
+// *bound describe the list the DFS consumes and *levels how many rows were expanded.
 static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, uint64_t n, int split, hipStream_t s,
-                         const uint32_t **tasks_out, const unsigned long long **n_dev, uint64_t *bound, int *levels) {
+                         const uint32_t **tasks_out, const unsigned long long **n_dev,
+                         uint64_t *bound, int *levels) {
     if (split > 0) {
         hipError_t e = launch_zero(d->xcount + 2, 1, s);
         if (e != hipSuccess) return fail_hip(e, "launch_zero");
@@ -227,11 +238,15 @@ static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, uint64
     uint64_t b = n;
     int j = 0;
     for (; j < split && d0 + j + 2 < N; ++j) {
-        const uint64_t nb = b * static_cast<uint64_t>(std::max(1, N - d0 - j));
+        hipError_t e = launch_child_count(in, b, in_dev, N, d->xcount + 3, s);
+        if (e != hipSuccess) return fail_hip(e, "launch_child_count");
+        unsigned long long nb = 0;
+        NQ_HIP(hipMemcpyAsync(&nb, d->xcount + 3, sizeof nb, hipMemcpyDeviceToHost, s));
+        NQ_HIP(hipStreamSynchronize(s));
         if (nb > kMaxExpandedTasks) break;
         int rc = ensure_xbuf(d, j & 1, nb);
         if (rc) return rc;
-        hipError_t e = launch_expand(in, b, in_dev, N, d->xbuf[j & 1], nb, &d->xcount[j & 1], &d->xcount[2], s);
+        e = launch_expand(in, b, in_dev, N, d->xbuf[j & 1], nb, &d->xcount[j & 1], &d->xcount[2], s);
         if (e != hipSuccess) return fail_hip(e, "launch_expand");
         in = d->xbuf[j & 1];
         in_dev = &d->xcount[j & 1];
@@ -291,9 +306,10 @@ int nq_config_default(int N, int d0, nq_config *cfg) {
     cfg->block = 128;
     cfg->stack_depth = std::max(1, frames_needed(N, d0));
     cfg->waves_per_cu = 0;
-    cfg->chunk = 1;
+    cfg->chunk = 16;
     cfg->grid = 0;
     cfg->split = 0;
+    cfg->push = 0;
     return NQ_OK;
 }
 
@@ -311,7 +327,7 @@ int nq_generate(int N, int rows, uint32_t *tasks, uint64_t cap, uint64_t *ntasks
 
 int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_config *cfg, uint64_t *d_result,
                     uint64_t *d_per_task, int count_nodes, void *stream) {
-    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..27 for the GPU kernel");
+    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..28 for the GPU kernel");
     if (ntasks && !d_tasks) return fail(NQ_ERR_ARG, "d_tasks is NULL");
     if (!d_result) return fail(NQ_ERR_ARG, "d_result is NULL");
     Device *d = nullptr;
@@ -341,6 +357,7 @@ int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_co
                  count_nodes != 0};
     a.ntasks_dev = n_dev;
     a.xflags = levels ? d->xcount + 2 : nullptr;
+    a.apush = c.push == NQ_PUSH_ALWAYS;
     hipError_t e = launch_count(a, s);
     if (e != hipSuccess) return fail_hip(e, "launch_count");
     return NQ_OK;
@@ -349,7 +366,7 @@ int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_co
 int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg, uint64_t *sum, uint64_t *per_task,
              uint64_t *nodes, nq_stats *stats) {
     const double t0 = now_ms();
-    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..27 for the GPU kernel");
+    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..28 for the GPU kernel");
     if (!sum) return fail(NQ_ERR_ARG, "sum is NULL");
     if (ntasks && !tasks) return fail(NQ_ERR_ARG, "tasks is NULL");
     Device *d = nullptr;
@@ -394,6 +411,7 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
                  per_task ? d->per_task : nullptr, d->result, nodes != nullptr, stats ? d->trace : nullptr};
     a.ntasks_dev = n_dev;
     a.xflags = levels ? d->xcount + 2 : nullptr;
+    a.apush = c.push == NQ_PUSH_ALWAYS;
     hipError_t e = launch_count(a, d->stream);
     if (e != hipSuccess) return fail_hip(e, "launch_count");
     NQ_HIP(hipEventRecord(d->ev1, d->stream));
@@ -428,17 +446,19 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     return NQ_OK;
 }
 
-// Rows nq_solve expands on the device: deep whole-board tasks get finer (shorter drain tail) and
-// need fewer LDS frames (more waves per CU).  Measured on N=20/21 with 6 rows pre-placed:
-// 14 free rows -> 1 row (1.45 -> 1.26 s), 15 free rows -> 2 rows (13.2 -> 10.4 s).
+// Rows nq_solve expands on the device (0..3): the DFS then starts (N + 4) / 2 rows above the
+// last row.  Shallower starts leave too few, too long tasks (drain tail, and more LDS frames per
+// lane, i.e. fewer waves per CU); deeper starts make tasks so short that claiming and loading
+// them dominates.  Measured best on one MI355X (kernel + expansion time): N=21 rows 6 -> 3
+// (8.34 s; 2: 8.42 s, 0: 10.31 s), N=20/6 -> 2 (0.98 s), N=19/6 -> 2, N=18/6 -> 1, N=17/5 -> 2,
+// N=16/4 -> 2; the rule gives each of these.
 static int auto_split(int N, int rows) {
-    const int free_rows = N - rows;
-    return free_rows >= 15 ? 2 : (free_rows >= 14 ? 1 : 0);
+    return std::max(0, std::min(3, N - rows - (N + 4) / 2));
 }
 
 int nq_solve_config(int N, int rows, nq_config *cfg) {
     if (!cfg) return fail(NQ_ERR_ARG, "cfg is NULL");
-    if (N < 1 || N > MAX_N || rows < 1 || rows > N) return fail(NQ_ERR_ARG, "N must be 1..27 and rows 1..N");
+    if (N < 1 || N > MAX_N || rows < 1 || rows > N) return fail(NQ_ERR_ARG, "N must be 1..28 and rows 1..N");
     const int split = auto_split(N, rows);
     nq_config_default(N, std::min(N, rows + split), cfg);
     cfg->split = split;
@@ -448,7 +468,7 @@ int nq_solve_config(int N, int rows, nq_config *cfg) {
 int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats) {
     const double t0 = now_ms();
     if (!total) return fail(NQ_ERR_ARG, "total is NULL");
-    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..27 for the GPU kernel");
+    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..28 for the GPU kernel");
     if (rows < 1 || rows > N) return fail(NQ_ERR_ARG, "rows must be 1..N");
     std::vector<uint32_t> tasks;
     generate_tasks(N, rows, tasks);

@@ -209,7 +209,9 @@ int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, n
         if (cfg->chunk) c.chunk = cfg->chunk;
         if (cfg->grid) c.grid = cfg->grid;
         if (cfg->stack_depth) c.stack_depth = cfg->stack_depth;
+        c.push = cfg->push;
     }
+    if (c.push < 0 || c.push > NQ_PUSH_ALWAYS) return fail(NQ_ERR_ARG, "push must be 0, 1 or 2");
     if (c.block < 64 || c.block > MAX_BLOCK || c.block % 64) return fail(NQ_ERR_ARG, "block must be 64..256, multiple of 64");
     if (c.chunk < 1 || c.chunk > 4096) return fail(NQ_ERR_ARG, "chunk must be 1..4096");
     // Per-task depth target: the stack depth + 1 when given, else 12 free rows (11 frames).
@@ -243,14 +245,15 @@ int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, n
         plan_free(p);
         return fail_hip(e, "batch plan allocation/upload");
     }
+    bool apush = false;
+    int bpc = 0;
+    e = batch_fit(c.block, c.stack_depth, c.waves_per_cu, c.push, &apush, &bpc);
+    if (e != hipSuccess || bpc <= 0) {
+        plan_free(p);
+        return e != hipSuccess ? fail_hip(e, "batch_occupancy") : fail(NQ_ERR_ARG, "batch does not fit on a CU");
+    }
+    c.push = apush ? NQ_PUSH_ALWAYS : NQ_PUSH_MASKED;
     if (c.grid == 0) {
-        int bpc = 0;
-        e = batch_occupancy(c.block, c.stack_depth, &bpc);
-        if (e != hipSuccess || bpc <= 0) {
-            plan_free(p);
-            return e != hipSuccess ? fail_hip(e, "batch_occupancy") : fail(NQ_ERR_ARG, "batch does not fit on a CU");
-        }
-        if (c.waves_per_cu > 0) bpc = std::max(1, std::min(bpc, c.waves_per_cu / (c.block / 64)));
         c.grid = cus * bpc;
         const uint64_t need = (p->ntasks + c.block - 1) / c.block;
         if (static_cast<uint64_t>(c.grid) > need) c.grid = static_cast<int>(std::max<uint64_t>(1, need));
@@ -258,7 +261,7 @@ int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, n
     p->cfg = c;
     // Capture zero-counts + reset + batched DFS into a graph (kernel nodes only).
     BatchArgs a{p->d_tasks, p->ntasks, p->d_counts, std::max<uint64_t>(1, p->ninst), c.block, c.grid, c.stack_depth,
-                c.chunk, p->d_state};
+                c.chunk, p->d_state, apush};
     e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         hipError_t le = launch_batch(a, p->stream);

@@ -65,7 +65,11 @@ long long cuda_n_queens(int N, int rows) {
     }
     nq::log_line("gpu [%d] start job, with %lld(%.2f) subproblems.\n", 0, static_cast<long long>(cnt), 1.0);
     uint64_t sum = 0;
-    int rc = nq_count(N, tasks.data(), cnt, nullptr, &sum, nullptr, nullptr, nullptr);
+    // The same device split as nq_solve (totals only; every child keeps its exact count).
+    nq_config solve_cfg;
+    nq_config c{};
+    if (nq_solve_config(N, rows, &solve_cfg) == NQ_OK) c.split = solve_cfg.split;
+    int rc = nq_count(N, tasks.data(), cnt, &c, &sum, nullptr, nullptr, nullptr);
     if (rc != NQ_OK) {
         fprintf(stderr, "cuda_n_queens: %s\n", nq_last_error());
         return -1;

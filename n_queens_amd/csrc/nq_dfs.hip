@@ -13,16 +13,18 @@
 //    Frames are 16 B {c, l, r, v} at stack[level][threadIdx.x]: lanes of a wave touch 64
 //    consecutive 16-B slots, so ds_write_b128 / ds_read_b128 are bank-conflict free whatever
 //    depth each lane is at.
-//  * The depth rides in bits 27..31 of c: placing queen p is c + p + (1 << 27) (p is never in
-//    c), and "the child is at the last row" is the single compare c >= (N-1) << 27.  The last
-//    row is counted by popcount of the child's candidates (0 or 1), as the reference does.
-//    A child with one last-row candidate is expanded once more instead of being cleared
-//    (that child is a solution, 1-2 % of nodes); this keeps the leaf path to compare + bcnt.
+//  * The columns word c has every bit from N up set: the candidates of a frame are then
+//    ~(l | r | c) (one v_bitop3, no board mask) and a completed board is c == ~0, counted with
+//    a compare and an add-with-carry.  Nothing in the iteration depends on N, so lanes of one
+//    wave may work on different board sizes (batch_kernel).
+//  * Two push forms: exec-masked (only frames with candidates left are written), or an
+//    unconditional write with sp advanced by stride * min(v, 1) (no exec switch, one spare
+//    stack level); the host takes the second when the spare level costs no occupancy.
 //  * A lane whose task is exhausted claims the next one at once, in the same loop (lane-
 //    granular refill), from a global cursor; claims are aggregated per wave and handed out in
 //    chunks through a wave-private LDS range.
 //  * Per-lane counts are 32-bit (a lane adds at most 1 per iteration) and folded into 64-bit
-//    every 2^30 iterations and at task end; the 64-bit sums are reduced per wave with DPP
+//    every 2^31 iterations and at task end; the 64-bit sums are reduced per wave with DPP
 //    shuffles, per workgroup through LDS, and added with one atomic per workgroup.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -35,8 +37,9 @@ namespace nq {
 
 namespace {
 
-constexpr uint32_t DEPTH_ONE = 1u << DEPTH_SHIFT;
-constexpr uint32_t FLUSH_EVERY = 1u << 29;  // hot-loop trips (4 iterations each) between folds
+// Hot-loop trips (8 iterations each) between folds of the 32-bit per-lane counters: at most
+// 8 * 2^28 = 2^31 increments between folds.
+constexpr uint32_t FLUSH_EVERY = 1u << 28;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
@@ -92,60 +95,74 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 
 // One DFS iteration for every live lane of the wave, hand-scheduled (gfx950).  The frame
 // {c, l, r, v} lives in v[20:23] (bound by the asm constraints below) so that it is both the
-// ds_write_b128 source and the ds_read_b128 destination without register moves.
+// ds_write_b128 source and the ds_read_b128 destination without register moves.  c holds the
+// occupied columns with every bit from N up set, so the candidates are ~(l | r | c) with no
+// board mask and a full board is c == ~0; the iteration does not depend on N.
 //
 //   p = v & -v; v ^= p                          lowest candidate, v = candidates left
-//   if (v != 0) { stack[sp] = {c,l,r,v}; sp += stride }      (exec-masked push)
-//   c += p + (1 << 27); l = (l|p) << 1; r = (r|p) >> 1
-//   v = last & ~(l|r) & ~c                     (v_bitop3 table 0x04: ~s0 & s1 & ~s2)
-//   if (c >= leaf) cnt += popcount(v)          (exec-masked)
+//   push {c,l,r,v} if v != 0                    (masked, or unconditional: see below)
+//   c |= p; l = (l|p) << 1; r = (r|p) >> 1
+//   v = ~(l | r | c)                            (v_bitop3 table 0x01)
+//   if (c == ~0) cnt += 1                       (compare + add with carry, no mask switch)
 //   if (v == 0) { if (sp != base) { sp -= stride; {c,l,r,v} = stack[sp] } else done |= lane }
 //
 // Only SALU instructions read the masks the VALU compares write, and every LDS read is waited
 // for before the iteration ends, so no hazard wait states are needed.  Lanes that become done
-// are removed from exec for the rest of the asm statement; [nd] counts expansions (node-
-// counting variant only).
-#define NQ_ITER(NODES_INSN)                                                                   \
+// are removed from exec for the rest of the asm statement.
+#define NQ_EXTRACT                                                                            \
     "v_sub_u32 %[t], 0, v23\n\t"                                                              \
     "v_and_b32 %[p], v23, %[t]\n\t"                                                           \
-    "v_xor_b32 v23, v23, %[p]\n\t" NODES_INSN                                                  \
+    "v_xor_b32 v23, v23, %[p]\n\t"
+
+// Push only the lanes whose frame keeps untried candidates (exec-masked).
+#define NQ_PUSH_MASKED                                                                        \
     "v_cmp_ne_u32 %[m1], 0, v23\n\t"                                                          \
     "s_and_saveexec_b64 %[tmp], %[m1]\n\t"                                                    \
     "ds_write_b128 %[sp], v[20:23]\n\t"                                                       \
     "v_add_u32 %[sp], %[stride], %[sp]\n\t"                                                   \
-    "s_mov_b64 exec, %[tmp]\n\t"                                                              \
-    "v_add3_u32 v20, v20, %[p], %[one]\n\t"                                                   \
+    "s_mov_b64 exec, %[tmp]\n\t"
+
+// Write every lane's frame and keep it (advance sp by stride * min(v, 1)) only where
+// candidates remain: no exec switch.  A lane whose every level is in use writes one level
+// above its top frame, so this form needs one spare level.
+#define NQ_PUSH_ALWAYS                                                                        \
+    "ds_write_b128 %[sp], v[20:23]\n\t"                                                       \
+    "v_min_u32 %[t], 1, v23\n\t"                                                              \
+    "v_mad_u32_u24 %[sp], %[t], %[stride], %[sp]\n\t"
+
+#define NQ_CHILD_POP                                                                          \
+    "v_or_b32 v20, v20, %[p]\n\t"                                                             \
     "v_or_b32 v21, v21, %[p]\n\t"                                                             \
     "v_lshlrev_b32 v21, 1, v21\n\t"                                                           \
     "v_or_b32 v22, v22, %[p]\n\t"                                                             \
     "v_lshrrev_b32 v22, 1, v22\n\t"                                                           \
-    "v_or_b32 %[t], v21, v22\n\t"                                                             \
-    "v_bitop3_b32 v23, %[t], %[last], v20 bitop3:4\n\t"                                       \
-    "v_cmp_le_u32 %[m1], %[leaf], v20\n\t"                                                    \
-    "s_and_saveexec_b64 %[tmp], %[m1]\n\t"                                                    \
-    "v_bcnt_u32_b32 %[cnt], v23, %[cnt]\n\t"                                                  \
-    "s_mov_b64 exec, %[tmp]\n\t"                                                              \
+    "v_bitop3_b32 v23, v21, v22, v20 bitop3:1\n\t"                                            \
+    "v_cmp_eq_u32 vcc, -1, v20\n\t"                                                           \
+    "v_addc_co_u32 %[cnt], vcc, 0, %[cnt], vcc\n\t"                                           \
     "v_cmp_eq_u32 %[m1], 0, v23\n\t"                                                          \
+    "v_cmp_eq_u32 %[m2], %[sp], %[base]\n\t"                                                  \
+    "s_and_b64 %[m2], %[m1], %[m2]\n\t"                                                       \
+    "s_andn2_b64 %[m1], %[m1], %[m2]\n\t"                                                     \
+    "s_or_b64 %[done], %[done], %[m2]\n\t"                                                    \
     "s_and_saveexec_b64 %[tmp], %[m1]\n\t"                                                    \
-    "v_cmp_ne_u32 %[m2], %[sp], %[base]\n\t"                                                  \
-    "s_and_b64 exec, exec, %[m2]\n\t"                                                         \
     "v_subrev_u32 %[sp], %[stride], %[sp]\n\t"                                                \
     "ds_read_b128 v[20:23], %[sp]\n\t"                                                        \
-    "s_andn2_b64 %[m1], %[m1], %[m2]\n\t"                                                     \
-    "s_or_b64 %[done], %[done], %[m1]\n\t"                                                    \
-    "s_andn2_b64 exec, %[tmp], %[m1]\n\t"                                                     \
+    "s_andn2_b64 exec, %[tmp], %[m2]\n\t"                                                     \
     "s_waitcnt lgkmcnt(0)\n\t"
 
-// The whole hot loop: four iterations per trip, leaving when some lane becomes done or after
-// [iters] trips (the caller folds the 32-bit counters then).  BC is the constraint of the board
-// mask and last-row threshold: "s" when the launch has one N, "v" when N varies per lane.
-#define NQ_HOT_ASM(NODES_INSN, BC)                                                            \
+#define NQ_ITER(PUSH, NODES_INSN) NQ_EXTRACT PUSH NODES_INSN NQ_CHILD_POP
+#define NQ_ITER2(PUSH, NODES_INSN) NQ_ITER(PUSH, NODES_INSN) NQ_ITER(PUSH, NODES_INSN)
+
+// The whole hot loop: eight iterations per trip, leaving when some lane becomes done or after
+// [iters] trips (the caller folds the 32-bit counters then).  [nd] counts expansions (node-
+// counting variant only).
+#define NQ_HOT_ASM(PUSH, NODES_INSN)                                                          \
     __asm__ volatile(                                                                         \
         "s_mov_b64 %[save], exec\n\t"                                                         \
         "s_mov_b64 exec, %[live]\n\t"                                                         \
         "s_mov_b64 %[done], 0\n"                                                              \
-        "nq_hot_%=:\n\t" NQ_ITER(NODES_INSN) NQ_ITER(NODES_INSN) NQ_ITER(NODES_INSN)         \
-        NQ_ITER(NODES_INSN)                                                                   \
+        "nq_hot_%=:\n\t" NQ_ITER2(PUSH, NODES_INSN) NQ_ITER2(PUSH, NODES_INSN)                \
+        NQ_ITER2(PUSH, NODES_INSN) NQ_ITER2(PUSH, NODES_INSN)                                 \
         "s_cmp_lg_u64 %[done], 0\n\t"                                                         \
         "s_cbranch_scc1 nq_exit_%=\n\t"                                                       \
         "s_sub_u32 %[iters], %[iters], 1\n\t"                                                 \
@@ -156,18 +173,40 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
         : "+{v20}"(c), "+{v21}"(l), "+{v22}"(r), "+{v23}"(v), [sp] "+v"(sp), [cnt] "+v"(cnt),   \
           [nd] "+v"(nd), [iters] "+s"(iters), [t] "=&v"(t), [p] "=&v"(p), [m1] "=&s"(m1),       \
           [m2] "=&s"(m2), [tmp] "=&s"(tmp), [save] "=&s"(save), [done] "=&s"(done)             \
-        : [live] "s"(live), [base] "v"(base), [stride] "s"(stride), [one] "s"(DEPTH_ONE),     \
-          [last] BC(last), [leaf] BC(leaf)                                                    \
-        : "memory", "scc")
+        : [live] "s"(live), [base] "v"(base), [stride] "s"(stride)                            \
+        : "memory", "scc", "vcc")
 
-template <bool NODES, bool PER_TASK>
+#define NQ_NODE_INC "v_add_u32 %[nd], 1, %[nd]\n\t"
+
+// Runs the hot loop for the lanes in `live` until one of them finishes its task (or the fold
+// interval ends).  APUSH selects the unconditional push (the stack has a spare level).
+template <bool NODES, bool APUSH>
+__device__ __forceinline__ void hot_loop(uint32_t &c, uint32_t &l, uint32_t &r, uint32_t &v, uint32_t &sp,
+                                         uint32_t &cnt, uint32_t &nd, uint64_t live, uint32_t base,
+                                         uint32_t stride) {
+    uint32_t t, p, iters = FLUSH_EVERY;
+    uint64_t m1, m2, tmp, save, done;
+    if (APUSH) {
+        if (NODES)
+            NQ_HOT_ASM(NQ_PUSH_ALWAYS, NQ_NODE_INC);
+        else
+            NQ_HOT_ASM(NQ_PUSH_ALWAYS, "");
+    } else {
+        if (NODES)
+            NQ_HOT_ASM(NQ_PUSH_MASKED, NQ_NODE_INC);
+        else
+            NQ_HOT_ASM(NQ_PUSH_MASKED, "");
+    }
+}
+
+template <bool NODES, bool PER_TASK, bool APUSH>
 __global__ void __launch_bounds__(MAX_BLOCK)
 dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint32_t chunk,
            uint32_t stack_depth, DevState *__restrict__ st, uint64_t *__restrict__ per_task,
            uint64_t *__restrict__ trace, const unsigned long long *__restrict__ ntasks_dev) {
     // Task count produced on the device (expansion), never beyond the buffer bound `ntasks`.
     if (ntasks_dev) ntasks = min(ntasks, static_cast<uint64_t>(*ntasks_dev));
-    extern __shared__ uint4 stack[];  // [stack_depth][blockDim.x]
+    extern __shared__ uint4 stack[];  // [stack_depth + APUSH][blockDim.x]
     __shared__ unsigned long long wq[MAX_WAVES_PER_BLOCK][2];
     __shared__ unsigned long long red[MAX_WAVES_PER_BLOCK][2];
 
@@ -178,7 +217,6 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
         wq[wave][1] = 0;
     }
     const uint32_t last = (1u << N) - 1u;
-    const uint32_t leaf = (N - 1u) << DEPTH_SHIFT;
     const uint32_t stride = blockDim.x * static_cast<uint32_t>(sizeof(uint4));
     const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(stack + threadIdx.x));
     uint32_t sp = base;
@@ -223,7 +261,7 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
                     if (PER_TASK) per_task[t] = 0;
                     continue;
                 }
-                c = c0 | (d0 << DEPTH_SHIFT);
+                c = c0 | ~last;
                 l = l0;
                 r = r0;
                 v = v0;
@@ -234,16 +272,8 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
         }
         const uint64_t live = __ballot(has);
         if (live == 0) break;
-        // Hot loop until a lane needs a new task; the trip bound keeps the 32-bit per-lane
-        // counters from wrapping (at most 4 * FLUSH_EVERY hits between folds).
-        {
-            uint32_t t, p, iters = FLUSH_EVERY;
-            uint64_t m1, m2, tmp, save, done;
-            if (NODES)
-                NQ_HOT_ASM("v_add_u32 %[nd], 1, %[nd]\n\t", "s");
-            else
-                NQ_HOT_ASM("", "s");
-        }
+        // Hot loop until a lane needs a new task.
+        hot_loop<NODES, APUSH>(c, l, r, v, sp, cnt, nd, live, base, stride);
         tsum += cnt;
         if (NODES) nodes += static_cast<uint64_t>(nd) - cnt;
         cnt = 0;
@@ -279,6 +309,7 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
 // the board mask and the last-row threshold) varies per lane, and every finished task adds its
 // count (doubled when `twice` is set: a mirror-symmetric half of an empty board) to counts[inst]
 // with one 64-bit atomic.  Same iteration as dfs_kernel.
+template <bool APUSH>
 __global__ void __launch_bounds__(MAX_BLOCK)
 batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, uint32_t stack_depth,
              DevState *__restrict__ st, unsigned long long *__restrict__ counts) {
@@ -295,7 +326,7 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
     const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(stack + threadIdx.x));
     uint32_t sp = base;
     uint32_t c = 0, l = 0, r = 0, v = 0, cnt = 0, nd = 0;
-    uint32_t last = 0, leaf = 0, inst = 0, weight = 1;
+    uint32_t inst = 0, weight = 1;
     uint64_t tsum = 0, acc = 0;
     bool has = true, own = false;
 
@@ -315,8 +346,7 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
                 const uint32_t N = w.w & 31u;
                 weight = 1u + ((w.w >> 5) & 1u);
                 inst = w.w >> 6;
-                last = (1u << N) - 1u;
-                leaf = (N - 1u) << DEPTH_SHIFT;
+                const uint32_t last = (1u << N) - 1u;
                 const uint32_t v0 = last & ~(w.x | w.y | w.z);
                 const uint32_t d0 = __popc(w.x);
                 if (d0 + 1u >= N || v0 == 0) {
@@ -332,7 +362,7 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
                     atomicOr(&st->error, 1ull);
                     continue;
                 }
-                c = w.x | (d0 << DEPTH_SHIFT);
+                c = w.x | ~last;
                 l = w.y;
                 r = w.z;
                 v = v0;
@@ -343,11 +373,7 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
         }
         const uint64_t live = __ballot(has);
         if (live == 0) break;
-        {
-            uint32_t t, p, iters = FLUSH_EVERY;
-            uint64_t m1, m2, tmp, save, done;
-            NQ_HOT_ASM("", "v");
-        }
+        hot_loop<false, APUSH>(c, l, r, v, sp, cnt, nd, live, base, stride);
         tsum += cnt;
         cnt = 0;
     }
@@ -440,14 +466,39 @@ __global__ void expand_kernel(const uint32_t *__restrict__ in, uint64_t n, const
     }
 }
 
-template <bool NODES, bool PER_TASK>
-void *kernel_ptr() {
-    return reinterpret_cast<void *>(&dfs_kernel<NODES, PER_TASK>);
+// *out += the number of outputs expand_kernel would write for these tasks (children of every
+// task, 1 for a full board).  *out must be zero on entry (launch_child_count zeroes it).
+__global__ void child_count_kernel(const uint32_t *__restrict__ in, uint64_t n,
+                                   const unsigned long long *__restrict__ n_dev, uint32_t N,
+                                   unsigned long long *__restrict__ out) {
+    if (n_dev) n = min(n, static_cast<uint64_t>(*n_dev));
+    const uint32_t last = (1u << N) - 1u;
+    uint64_t k = 0;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t c = in[3 * i];
+        k += c == last ? 1u : __popc(last & ~(c | in[3 * i + 1] | in[3 * i + 2]));
+    }
+    k = wave_sum_u64(k);
+    if ((threadIdx.x & 63u) == 0 && k) atomicAdd(out, static_cast<unsigned long long>(k));
 }
 
-const void *select_kernel(bool nodes, bool per_task) {
-    return nodes ? (per_task ? kernel_ptr<true, true>() : kernel_ptr<true, false>())
-                 : (per_task ? kernel_ptr<false, true>() : kernel_ptr<false, false>());
+const void *select_kernel(bool nodes, bool per_task, bool apush) {
+    const void *k[2][2][2] = {
+        {{reinterpret_cast<const void *>(&dfs_kernel<false, false, false>),
+          reinterpret_cast<const void *>(&dfs_kernel<false, false, true>)},
+         {reinterpret_cast<const void *>(&dfs_kernel<false, true, false>),
+          reinterpret_cast<const void *>(&dfs_kernel<false, true, true>)}},
+        {{reinterpret_cast<const void *>(&dfs_kernel<true, false, false>),
+          reinterpret_cast<const void *>(&dfs_kernel<true, false, true>)},
+         {reinterpret_cast<const void *>(&dfs_kernel<true, true, false>),
+          reinterpret_cast<const void *>(&dfs_kernel<true, true, true>)}}};
+    return k[nodes][per_task][apush];
+}
+
+const void *select_batch_kernel(bool apush) {
+    return apush ? reinterpret_cast<const void *>(&batch_kernel<true>)
+                 : reinterpret_cast<const void *>(&batch_kernel<false>);
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel (gfx950 has 160 KiB per CU).
@@ -458,12 +509,24 @@ hipError_t allow_lds(const void *k, size_t lds) {
 
 }  // namespace
 
-hipError_t batch_occupancy(int block, int stack_depth, int *blocks_per_cu) {
-    const void *k = reinterpret_cast<const void *>(&batch_kernel);
-    const size_t lds = dfs_lds_bytes(block, stack_depth);
+hipError_t batch_occupancy(int block, int stack_depth, bool apush, int *blocks_per_cu) {
+    const void *k = select_batch_kernel(apush);
+    const size_t lds = dfs_lds_bytes(block, stack_depth + (apush ? 1 : 0));
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, block, lds);
+}
+
+hipError_t batch_fit(int block, int stack_depth, int waves_per_cu, int push, bool *apush, int *blocks_per_cu) {
+    int occ[2] = {0, 0};
+    hipError_t e = batch_occupancy(block, stack_depth, false, &occ[0]);
+    if (e == hipSuccess) e = batch_occupancy(block, stack_depth, true, &occ[1]);
+    if (e != hipSuccess) return e;
+    if (waves_per_cu > 0)
+        for (int &o : occ) o = std::min(o, std::max(1, waves_per_cu / (block / 64)));
+    *apush = push == 2 || (push == 0 && occ[1] > 0 && occ[1] >= occ[0]);
+    *blocks_per_cu = occ[*apush ? 1 : 0];
+    return hipSuccess;
 }
 
 hipError_t launch_batch(const BatchArgs &a, hipStream_t stream) {
@@ -473,23 +536,26 @@ hipError_t launch_batch(const BatchArgs &a, hipStream_t stream) {
     reset_state_kernel<<<1, 64, 0, stream>>>(a.state);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const size_t lds = dfs_lds_bytes(a.block, a.stack_depth);
-    e = allow_lds(reinterpret_cast<const void *>(&batch_kernel), lds);
+    const size_t lds = dfs_lds_bytes(a.block, a.stack_depth + (a.apush ? 1 : 0));
+    e = allow_lds(select_batch_kernel(a.apush), lds);
     if (e != hipSuccess) return e;
-    batch_kernel<<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, static_cast<uint32_t>(a.chunk),
-                                                  static_cast<uint32_t>(a.stack_depth), a.state,
-                                                  reinterpret_cast<unsigned long long *>(a.counts));
+    const uint32_t chunk = static_cast<uint32_t>(a.chunk), sd = static_cast<uint32_t>(a.stack_depth);
+    auto *counts = reinterpret_cast<unsigned long long *>(a.counts);
+    if (a.apush)
+        batch_kernel<true><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, chunk, sd, a.state, counts);
+    else
+        batch_kernel<false><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, chunk, sd, a.state, counts);
     return hipGetLastError();
 }
 
-size_t dfs_lds_bytes(int block, int stack_depth) {
-    return static_cast<size_t>(block) * static_cast<size_t>(stack_depth) * sizeof(uint4);
+size_t dfs_lds_bytes(int block, int levels) {
+    return static_cast<size_t>(block) * static_cast<size_t>(levels) * sizeof(uint4);
 }
 
-hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task,
+hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task, bool apush,
                          int *blocks_per_cu) {
-    const void *k = select_kernel(count_nodes, per_task);
-    const size_t lds = dfs_lds_bytes(block, stack_depth);
+    const void *k = select_kernel(count_nodes, per_task, apush);
+    const size_t lds = dfs_lds_bytes(block, stack_depth + (apush ? 1 : 0));
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, block, lds);
@@ -511,31 +577,37 @@ hipError_t launch_expand(const uint32_t *in, uint64_t n_max, const unsigned long
     return hipGetLastError();
 }
 
+hipError_t launch_child_count(const uint32_t *in, uint64_t n_max, const unsigned long long *n_dev, int N,
+                              unsigned long long *out, hipStream_t stream) {
+    zero_u64_kernel<<<1, 64, 0, stream>>>(out, 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint64_t blocks = std::min<uint64_t>((n_max + 255) / 256, 4096);
+    child_count_kernel<<<static_cast<unsigned>(std::max<uint64_t>(1, blocks)), 256, 0, stream>>>(
+        in, n_max, n_dev, static_cast<uint32_t>(N), out);
+    return hipGetLastError();
+}
+
 hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
     reset_state_kernel<<<1, 64, 0, stream>>>(a.state);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const size_t lds = dfs_lds_bytes(a.block, a.stack_depth);
-    e = allow_lds(select_kernel(a.count_nodes, a.per_task != nullptr), lds);
+    const size_t lds = dfs_lds_bytes(a.block, a.stack_depth + (a.apush ? 1 : 0));
+    const void *k = select_kernel(a.count_nodes, a.per_task != nullptr, a.apush);
+    e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
-    const uint32_t N = static_cast<uint32_t>(a.N);
-    const uint32_t chunk = static_cast<uint32_t>(a.chunk);
-    const uint32_t sd = static_cast<uint32_t>(a.stack_depth);
-    if (a.count_nodes) {
-        if (a.per_task)
-            dfs_kernel<true, true><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                      a.per_task, a.trace, a.ntasks_dev);
-        else
-            dfs_kernel<true, false><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                       a.per_task, a.trace, a.ntasks_dev);
-    } else {
-        if (a.per_task)
-            dfs_kernel<false, true><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                       a.per_task, a.trace, a.ntasks_dev);
-        else
-            dfs_kernel<false, false><<<a.grid, a.block, lds, stream>>>(a.tasks, a.ntasks, N, chunk, sd, a.state,
-                                                                        a.per_task, a.trace, a.ntasks_dev);
-    }
+    uint32_t N = static_cast<uint32_t>(a.N);
+    uint32_t chunk = static_cast<uint32_t>(a.chunk);
+    uint32_t sd = static_cast<uint32_t>(a.stack_depth);
+    const uint32_t *tasks = a.tasks;
+    uint64_t ntasks = a.ntasks;
+    DevState *state = a.state;
+    uint64_t *per_task = a.per_task;
+    uint64_t *trace = a.trace;
+    const unsigned long long *ntasks_dev = a.ntasks_dev;
+    void *args[] = {&tasks, &ntasks, &N, &chunk, &sd, &state, &per_task, &trace, &ntasks_dev};
+    e = hipLaunchKernel(k, dim3(a.grid), dim3(a.block), args, lds, stream);
+    if (e != hipSuccess) return e;
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (a.result) {

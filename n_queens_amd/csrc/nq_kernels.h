@@ -6,11 +6,9 @@
 
 namespace nq {
 
-// Bit layout of the "cur" word the DFS keeps per lane: board columns in bits 0..N-1 and the
-// number of queens placed (the depth) in bits DEPTH_SHIFT..31, so that placing a queen is one
-// add (cur + p + DEPTH_ONE) and the last-row test is one unsigned compare.
-constexpr int DEPTH_SHIFT = 27;
-constexpr int MAX_N = 27;  // board bits must stay below DEPTH_SHIFT
+// Largest board.  The DFS keeps the columns word with every bit from N up set (the
+// candidates are ~(left | right | cur)); the batched task word packs N into 5 bits.
+constexpr int MAX_N = 28;
 constexpr int MAX_BLOCK = 256;
 constexpr int MAX_WAVES_PER_BLOCK = MAX_BLOCK / 64;
 
@@ -37,6 +35,7 @@ struct LaunchArgs {
     uint64_t *trace = nullptr;  // nullable: per wave {start, end} wall-clock ticks (grid * block/64 * 2)
     const unsigned long long *ntasks_dev = nullptr;  // nullable: task count on the device (<= ntasks)
     const unsigned long long *xflags = nullptr;      // nullable: expansion flags folded into result[2]
+    bool apush = false;  // unconditional push (the launch allocates stack_depth + 1 levels)
 };
 
 struct BatchArgs {
@@ -46,11 +45,16 @@ struct BatchArgs {
     uint64_t ninst;
     int block, grid, stack_depth, chunk;
     DevState *state;
+    bool apush = false;  // unconditional push (stack_depth + 1 levels)
 };
 
 // Enqueue: zero counts, reset state, batched DFS.  Capturable into a hipGraph (kernels only).
 hipError_t launch_batch(const BatchArgs &a, hipStream_t stream);
-hipError_t batch_occupancy(int block, int stack_depth, int *blocks_per_cu);
+hipError_t batch_occupancy(int block, int stack_depth, bool apush, int *blocks_per_cu);
+// Push form and resident workgroups per CU for a batched launch: push 0 takes the
+// unconditional push when its spare stack level costs no workgroup, 1 masked, 2 unconditional;
+// waves_per_cu > 0 caps the workgroups.
+hipError_t batch_fit(int block, int stack_depth, int waves_per_cu, int push, bool *apush, int *blocks_per_cu);
 
 // Enqueue: reset state, DFS, (optional) fold into result.  Returns a hipError_t.
 hipError_t launch_count(const LaunchArgs &a, hipStream_t stream);
@@ -61,12 +65,18 @@ hipError_t launch_count(const LaunchArgs &a, hipStream_t stream);
 hipError_t launch_expand(const uint32_t *in, uint64_t n_max, const unsigned long long *n_dev, int N, uint32_t *out,
                          uint64_t cap, unsigned long long *out_n, unsigned long long *flags, hipStream_t stream);
 
+// *out = the number of tasks launch_expand would produce from these (zeroed by the launch).
+hipError_t launch_child_count(const uint32_t *in, uint64_t n_max, const unsigned long long *n_dev, int N,
+                              unsigned long long *out, hipStream_t stream);
+
 // Zero n 64-bit words with a kernel (graph-capturable, unlike a memset node).
 hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t stream);
 
-// Resident workgroups per CU for the DFS kernel at (block, stack_depth).
-hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task, int *blocks_per_cu);
+// Resident workgroups per CU for the DFS kernel at (block, stack_depth, push form).
+hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task, bool apush,
+                         int *blocks_per_cu);
 
-size_t dfs_lds_bytes(int block, int stack_depth);
+// Dynamic LDS of a launch with `levels` stack levels.
+size_t dfs_lds_bytes(int block, int levels);
 
 }  // namespace nq

@@ -161,16 +161,17 @@ static hipError_t launch_slot(nq_server *S, Slot &s) {
     hipError_t e = n ? hipMemcpyAsync(s.d_tasks, s.h_tasks, n * sizeof(uint4), hipMemcpyHostToDevice, s.stream)
                      : hipSuccess;
     int grid = 0;
+    bool apush = false;
     if (e == hipSuccess) {
         int bpc = 0;
-        e = batch_occupancy(S->cfg.block, frames, &bpc);
+        e = batch_fit(S->cfg.block, frames, 0, 0, &apush, &bpc);
         grid = std::max(1, S->cus * std::max(1, bpc));
         const uint64_t need = (n + S->cfg.block - 1) / S->cfg.block;
         if (static_cast<uint64_t>(grid) > need) grid = static_cast<int>(std::max<uint64_t>(1, need));
     }
     if (e == hipSuccess) {
         BatchArgs a{s.d_tasks, n, s.d_counts, std::max<uint64_t>(1, s.slices.size()), S->cfg.block, grid, frames,
-                    S->cfg.chunk, s.d_state};
+                    S->cfg.chunk, s.d_state, apush};
         e = launch_batch(a, s.stream);
     }
     if (e == hipSuccess)

@@ -22,7 +22,7 @@ def test_mix_generator_is_seeded_and_valid():
         assert oracle.encode_prefix(N, cols) is not None
 
 
-@pytest.mark.parametrize("bad", [(8, [0, 1]), (8, [0, 8]), (28, [0]), (5, [0, 2, 4, 1, 3, 0])])
+@pytest.mark.parametrize("bad", [(8, [0, 1]), (8, [0, 8]), (29, [0]), (5, [0, 2, 4, 1, 3, 0])])
 def test_batch_rejects_invalid_instances(bad):
     # Validation happens before any device work, so this also runs without a GPU.
     with pytest.raises(nq.NQError) as e:

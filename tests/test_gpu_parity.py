@@ -68,6 +68,8 @@ def test_node_count_vs_oracle(N, rows):
 @pytest.mark.parametrize("cfg", [
     nq.Config(block=64, chunk=1), nq.Config(block=128, chunk=8), nq.Config(block=256, chunk=64),
     nq.Config(block=64, stack_depth=24), nq.Config(block=192, chunk=3, grid=7), nq.Config(waves_per_cu=4),
+    nq.Config(push=nq.NQ_PUSH_MASKED), nq.Config(push=nq.NQ_PUSH_ALWAYS, chunk=5),
+    nq.Config(push=nq.NQ_PUSH_ALWAYS, block=256, waves_per_cu=8),
 ])
 def test_launch_configs_same_counts(cfg, ref_vectors):
     t = ref_vectors["N13_r5_tasks"]
@@ -184,3 +186,62 @@ def test_batch_chunked_claims(chunk):
     for (N, cols), got in zip(mix, b):
         if N <= 15:
             assert got == oracle.count_subtree(N, *oracle.encode_prefix(N, cols)), (N, cols)
+
+
+@pytest.mark.parametrize("push", [nq.NQ_PUSH_MASKED, nq.NQ_PUSH_ALWAYS])
+def test_push_forms_exact(push, published):
+    """Both DFS push forms (exec-masked and unconditional with a spare level) give the same
+    totals, per-task counts and node counts, also where the stack is used to its last level."""
+    for N, rows in ((14, 3), (16, 4), (18, 5)):
+        total, _ = nq.solve(N, rows, nq.Config(push=push, split=-1))
+        assert total == published["solutions"][N], (N, rows, push)
+    N, rows = 13, 2
+    t = nq.generate(N, rows)
+    r = nq.count(N, t, cfg=nq.Config(push=push), per_task=True, nodes=True)
+    assert np.array_equal(r.per_task, oracle.count_tasks(N, t, per_task=True)[1])
+    assert r.nodes == oracle.count_tasks_nodes(N, t)
+    # Tasks at depth 0 (the whole board): every lane may fill all N-1 levels.
+    t0 = np.zeros((1, 3), np.uint32)
+    for N in (13, 14, 15):
+        assert nq.count(N, t0, cfg=nq.Config(push=push)).sum == published["solutions"][N]
+
+
+def test_n28_deep_prefixes():
+    """N = 28 (the reference's largest board): prefixes 16 rows deep, counted by nq_count and by
+    the batched kernel, against the oracle."""
+    N = 28
+    rng = np.random.default_rng(28)
+    prefixes = []
+    while len(prefixes) < 64:
+        cols = _random_prefix(rng, N, 16)
+        if cols is not None:
+            prefixes.append(cols)
+    t = np.array([oracle.encode_prefix(N, cols) for cols in prefixes], dtype=np.uint32)
+    expect = np.array([oracle.count_subtree(N, *x) for x in t], dtype=np.uint64)
+    assert int(expect.sum()) > 0
+    r = nq.count(N, t, per_task=True)
+    assert np.array_equal(r.per_task, expect)
+    counts, _ = nq.batch_count([(N, cols) for cols in prefixes])
+    assert np.array_equal(counts, expect)
+
+
+def _random_prefix(rng, N, k):
+    """A random non-attacking placement of k queens in the first k rows (None if stuck)."""
+    cols = []
+    for row in range(k):
+        ok = [c for c in range(N) if all(c != q and abs(c - q) != row - i for i, q in enumerate(cols))]
+        if not ok:
+            return None
+        cols.append(int(rng.choice(ok)))
+    return cols
+
+
+def test_tuner_small_board(tmp_path, published):
+    """The autotuner times every candidate, agrees on the total and caches the winner."""
+    from n_queens_amd import tune
+    path = str(tmp_path / "tune.json")
+    win, res = tune.tune(13, 3, quick=True, path=path, log=lambda s: None)
+    assert len(res) == len(tune.candidates(13, 3, quick=True))
+    assert tune.tuned_config(13, 3, path=path) == win
+    total, _ = nq.solve(13, 3, win)
+    assert total == published["solutions"][13]

@@ -112,3 +112,14 @@ def test_cli_no_gpu_path():
     assert re.search(r"Use [\d.]+ms to generate 66 subproblems!", lines[2])
     assert "Failed to find any gpu!" in p.stdout
     assert re.search(r"cuda 13 queens result -1, calc time: \[[\d.]+ms\]", p.stdout)
+
+
+def test_tuner_candidates_respect_depth():
+    """The autotuner only proposes splits that leave the DFS two rows to place."""
+    from n_queens_amd import tune
+    for N, rows in ((21, 6), (12, 9), (8, 5), (6, 5)):
+        cands = tune.candidates(N, rows)
+        assert cands
+        assert all(rows + c.split + 2 < N or c.split == 0 for c in cands)
+        assert {c.push for c in cands} == {nq.NQ_PUSH_MASKED, nq.NQ_PUSH_ALWAYS}
+    assert len(tune.candidates(21, 6, quick=True)) == 8
