@@ -217,39 +217,45 @@ static int ensure_xbuf(Device *d, int i, uint64_t n) {
     return NQ_OK;
 }
 
-// Device expansion budget: the levels stop before a buffer would exceed this many tasks.
-static constexpr uint64_t kMaxExpandedTasks = 2ull << 30;  // 24 GiB of 12-B tasks
+// Device expansion budget: a level is expanded only if its buffer (12 B per task) fits in 40 %
+// of the device memory free at the time, and never beyond 8 G tasks.
+static uint64_t max_expanded_tasks() {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 2ull << 30;
+    return std::min<uint64_t>(8ull << 30, static_cast<uint64_t>(free_b * 0.4) / 12);
+}
 
-// Enqueue up to `split` one-row expansions of (tasks, n) on the device (totals only; a full
-// board is carried over).  The child count of every level is reduced on the device and read
-// back first (one stream synchronization per row) so that each level's buffer has its exact
-// size (the worst case, every task having N - depth children, is ~10x the real count per row);
-// a level larger than kMaxExpandedTasks ends the split there.  On return *tasks_out / *n_dev /
-// *bound describe the list the DFS consumes and *levels how many rows were expanded.
-static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, uint64_t n, int split, hipStream_t s,
-                         const uint32_t **tasks_out, const unsigned long long **n_dev,
-                         uint64_t *bound, int *levels) {
-    if (split > 0) {
-        hipError_t e = launch_zero(d->xcount + 2, 1, s);
-        if (e != hipSuccess) return fail_hip(e, "launch_zero");
-    }
+// Enqueue up to `split` one-row expansions of (tasks, n) on the device (a full board is carried
+// over; order is free).  The input count is n, or min(n, *n_dev_in) when n_dev_in is set.  The
+// child count of every level is reduced on the device and read back first (one stream
+// synchronization per row) so that each level's buffer has its exact size (the worst case,
+// every task having N - depth children, is ~10x the real count per row); a level larger than
+// max_expanded_tasks() ends the expansion there.  Level j writes expansion buffer
+// (first + j) & 1, so a second call can continue from the first one's output.  Rows are
+// expanded while the DFS keeps two rows to place (to_full: while the depth is below N).  The
+// caller zeroes the overflow flag (xcount[2]).  On return *tasks_out / *n_dev / *bound describe
+// the expanded list (*bound is its exact size) and *levels how many rows were expanded.
+static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, const unsigned long long *n_dev_in,
+                         uint64_t n, int split, int first, bool to_full, hipStream_t s, const uint32_t **tasks_out,
+                         const unsigned long long **n_dev, uint64_t *bound, int *levels) {
     const uint32_t *in = tasks;
-    const unsigned long long *in_dev = nullptr;
+    const unsigned long long *in_dev = n_dev_in;
     uint64_t b = n;
     int j = 0;
-    for (; j < split && d0 + j + 2 < N; ++j) {
+    for (; j < split && (to_full ? d0 + j < N : d0 + j + 2 < N); ++j) {
+        const int buf = (first + j) & 1;
         hipError_t e = launch_child_count(in, b, in_dev, N, d->xcount + 3, s);
         if (e != hipSuccess) return fail_hip(e, "launch_child_count");
         unsigned long long nb = 0;
         NQ_HIP(hipMemcpyAsync(&nb, d->xcount + 3, sizeof nb, hipMemcpyDeviceToHost, s));
         NQ_HIP(hipStreamSynchronize(s));
-        if (nb > kMaxExpandedTasks) break;
-        int rc = ensure_xbuf(d, j & 1, nb);
+        if (nb > max_expanded_tasks()) break;
+        int rc = ensure_xbuf(d, buf, nb);
         if (rc) return rc;
-        e = launch_expand(in, b, in_dev, N, d->xbuf[j & 1], nb, &d->xcount[j & 1], &d->xcount[2], s);
+        e = launch_expand(in, b, in_dev, N, d->xbuf[buf], nb, &d->xcount[buf], &d->xcount[2], s);
         if (e != hipSuccess) return fail_hip(e, "launch_expand");
-        in = d->xbuf[j & 1];
-        in_dev = &d->xcount[j & 1];
+        in = d->xbuf[buf];
+        in_dev = &d->xcount[buf];
         b = nb;
     }
     *tasks_out = in;
@@ -263,6 +269,123 @@ static int min_depth(int N, const uint32_t *tasks, uint64_t ntasks) {
     int d0 = N;
     for (uint64_t i = 0; i < ntasks; ++i) d0 = std::min(d0, __builtin_popcount(tasks[3 * i]));
     return d0;
+}
+
+// Rows nq_solve expands on the device (0..3): the DFS then starts (N + 4) / 2 rows above the
+// last row.  Shallower starts leave too few, too long tasks (drain tail, and more LDS frames per
+// lane, i.e. fewer waves per CU); deeper starts make tasks so short that claiming and loading
+// them dominates.  Measured best on one MI355X (kernel + expansion time): N=21 rows 6 -> 3
+// (8.34 s; 2: 8.42 s, 0: 10.31 s), N=20/6 -> 2 (0.98 s), N=19/6 -> 2, N=18/6 -> 1, N=17/5 -> 2,
+// N=16/4 -> 2; the rule gives each of these.
+static int auto_split(int N, int rows) {
+    return std::max(0, std::min(3, N - rows - (N + 4) / 2));
+}
+
+
+// Whole-board count with the task list generated on the device.  The reference's tasks at
+// depth `rows` (src/n_queens.cpp:74-113) are every legal continuation of its tasks at depth
+// min(rows, 2): only row 0 (low half) and, for odd N, rows 0-1 (middle column, then the low
+// side) are restricted.  The host generates those few seeds; the device expands them to depth
+// `rows` (exact sizes, order free: the same multiset of tasks), then by the DFS split, and
+// counts.  on_generated (nullable) is called once the depth-`rows` list exists, with its task
+// count and the generation time.
+int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats,
+                    void (*on_generated)(void *ctx, uint64_t ntasks, double ms), void *ctx) {
+    const double t0 = now_ms();
+    if (!total) return fail(NQ_ERR_ARG, "total is NULL");
+    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..28 for the GPU kernel");
+    if (rows < 1 || rows > N) return fail(NQ_ERR_ARG, "rows must be 1..N");
+    Device *d = nullptr;
+    int rc = current_device(&d);
+    if (rc) return rc;
+    const int seed_rows = std::min(rows, 2);
+    std::vector<uint32_t> seeds;
+    generate_tasks(N, seed_rows, seeds);
+    const uint64_t nseeds = seeds.size() / 3;
+    nq_config in = cfg ? *cfg : nq_config{};
+    const int split = (!cfg || in.split < 0) ? auto_split(N, rows) : in.split;
+
+    std::lock_guard<std::mutex> g(d->mu);
+    rc = ensure_tasks(d, nseeds);
+    if (rc) return rc;
+    const double t1 = now_ms();
+    if (nseeds)
+        NQ_HIP(hipMemcpyAsync(d->tasks, seeds.data(), seeds.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              d->stream));
+    NQ_HIP(hipMemsetAsync(d->result, 0, 3 * sizeof(uint64_t), d->stream));
+    hipError_t e = launch_zero(d->xcount + 2, 1, d->stream);
+    if (e != hipSuccess) return fail_hip(e, "launch_zero");
+    NQ_HIP(hipStreamSynchronize(d->stream));
+    const double t2 = now_ms();
+
+    // Generation: seeds -> depth `rows`.
+    const uint32_t *t = d->tasks;
+    const unsigned long long *n_dev = nullptr;
+    uint64_t n = nseeds;
+    int gen = 0;
+    if (nseeds) {
+        rc = enqueue_split(d, N, seed_rows, d->tasks, nullptr, nseeds, rows - seed_rows, 0, true, d->stream, &t,
+                           &n_dev, &n, &gen);
+        if (rc) return rc;
+    }
+    NQ_HIP(hipStreamSynchronize(d->stream));
+    const double t3 = now_ms();
+    const uint64_t ntasks_rows = n;  // the reference's task count for (N, rows)
+    if (on_generated) on_generated(ctx, ntasks_rows, t3 - t0 - (t2 - t1));
+
+    // Split for the DFS, then count.
+    NQ_HIP(hipEventRecord(d->ev0, d->stream));
+    int lev = 0;
+    if (split && n) {
+        rc = enqueue_split(d, N, seed_rows + gen, t, n_dev, n, split, gen & 1, false, d->stream, &t, &n_dev, &n,
+                           &lev);
+        if (rc) return rc;
+    }
+    // cfg->stack_depth bounds the depth-`rows` tasks, as for nq_count; the split takes lev rows off.
+    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - lev);
+    in.split = 0;
+    nq_config c;
+    rc = resolve_config(d, N, std::min(N, seed_rows + gen + lev), n, &in, false, false, &c);
+    if (rc) return rc;
+    const size_t nwaves = static_cast<size_t>(c.grid) * (c.block / 64);
+    if (stats) {
+        rc = ensure_trace(d, 2 * nwaves);
+        if (rc) return rc;
+    }
+    LaunchArgs a{t, n, N, c.block, c.grid, c.stack_depth, c.chunk, d->state, nullptr, d->result, false,
+                 stats ? d->trace : nullptr};
+    a.ntasks_dev = n_dev;
+    a.xflags = (gen + lev) ? d->xcount + 2 : nullptr;
+    a.apush = c.push == NQ_PUSH_ALWAYS;
+    e = launch_count(a, d->stream);
+    if (e != hipSuccess) return fail_hip(e, "launch_count");
+    NQ_HIP(hipEventRecord(d->ev1, d->stream));
+    uint64_t res[3] = {0, 0, 0};
+    DevState st;
+    NQ_HIP(hipMemcpyAsync(res, d->result, sizeof res, hipMemcpyDeviceToHost, d->stream));
+    NQ_HIP(hipMemcpyAsync(&st, d->state, sizeof st, hipMemcpyDeviceToHost, d->stream));
+    std::vector<uint64_t> tr;
+    if (stats) {
+        tr.resize(2 * nwaves);
+        NQ_HIP(hipMemcpyAsync(tr.data(), d->trace, tr.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, d->stream));
+    }
+    NQ_HIP(hipStreamSynchronize(d->stream));
+    if (res[2] & 2) return fail(NQ_ERR_STATE, "device expansion overflowed its buffer");
+    if (st.error || (res[2] & 1)) return fail(NQ_ERR_DEPTH, "a task needed more stack frames than configured");
+    *total = 2 * res[0];  // symmetry fold, src/n_queens_cuda.cu:181-183
+    if (stats) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, d->ev0, d->ev1);
+        stats->ntasks = ntasks_rows;
+        stats->nodes = 0;
+        stats->gen_ms = t3 - t0 - (t2 - t1);
+        stats->h2d_ms = t2 - t1;
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t0;
+        trace_stats(tr, d->wall_khz, &stats->busy_frac, &stats->tail_ms);
+        stats->waves = static_cast<uint64_t>(nwaves);
+    }
+    return NQ_OK;
 }
 
 }  // namespace nq
@@ -345,7 +468,9 @@ int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_co
     uint64_t bound = ntasks;
     int levels = 0;
     if (split) {
-        rc = enqueue_split(d, N, d0, d_tasks, ntasks, split, s, &t, &n_dev, &bound, &levels);
+        hipError_t e = launch_zero(d->xcount + 2, 1, s);
+        if (e != hipSuccess) return fail_hip(e, "launch_zero");
+        rc = enqueue_split(d, N, d0, d_tasks, nullptr, ntasks, split, 0, false, s, &t, &n_dev, &bound, &levels);
         if (rc) return rc;
     }
     nq_config in = cfg ? *cfg : nq_config{};
@@ -394,7 +519,10 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     uint64_t bound = ntasks;
     int levels = 0;
     if (split) {
-        rc = enqueue_split(d, N, d0, d->tasks, ntasks, split, d->stream, &t, &n_dev, &bound, &levels);
+        hipError_t e = launch_zero(d->xcount + 2, 1, d->stream);
+        if (e != hipSuccess) return fail_hip(e, "launch_zero");
+        rc = enqueue_split(d, N, d0, d->tasks, nullptr, ntasks, split, 0, false, d->stream, &t, &n_dev, &bound,
+                           &levels);
         if (rc) return rc;
     }
     nq_config in = cfg ? *cfg : nq_config{};
@@ -446,16 +574,6 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     return NQ_OK;
 }
 
-// Rows nq_solve expands on the device (0..3): the DFS then starts (N + 4) / 2 rows above the
-// last row.  Shallower starts leave too few, too long tasks (drain tail, and more LDS frames per
-// lane, i.e. fewer waves per CU); deeper starts make tasks so short that claiming and loading
-// them dominates.  Measured best on one MI355X (kernel + expansion time): N=21 rows 6 -> 3
-// (8.34 s; 2: 8.42 s, 0: 10.31 s), N=20/6 -> 2 (0.98 s), N=19/6 -> 2, N=18/6 -> 1, N=17/5 -> 2,
-// N=16/4 -> 2; the rule gives each of these.
-static int auto_split(int N, int rows) {
-    return std::max(0, std::min(3, N - rows - (N + 4) / 2));
-}
-
 int nq_solve_config(int N, int rows, nq_config *cfg) {
     if (!cfg) return fail(NQ_ERR_ARG, "cfg is NULL");
     if (N < 1 || N > MAX_N || rows < 1 || rows > N) return fail(NQ_ERR_ARG, "N must be 1..28 and rows 1..N");
@@ -466,25 +584,7 @@ int nq_solve_config(int N, int rows, nq_config *cfg) {
 }
 
 int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats) {
-    const double t0 = now_ms();
-    if (!total) return fail(NQ_ERR_ARG, "total is NULL");
-    if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..28 for the GPU kernel");
-    if (rows < 1 || rows > N) return fail(NQ_ERR_ARG, "rows must be 1..N");
-    std::vector<uint32_t> tasks;
-    generate_tasks(N, rows, tasks);
-    const double t1 = now_ms();
-    const uint64_t n = tasks.size() / 3;
-    uint64_t sum = 0;
-    nq_config c = cfg ? *cfg : nq_config{};
-    if (!cfg || c.split < 0) c.split = auto_split(N, rows);
-    int rc = nq_count(N, tasks.data(), n, &c, &sum, nullptr, nullptr, stats);
-    if (rc) return rc;
-    *total = 2 * sum;  // symmetry fold, src/n_queens_cuda.cu:181-183
-    if (stats) {
-        stats->gen_ms = t1 - t0;
-        stats->total_ms = now_ms() - t0;
-    }
-    return NQ_OK;
+    return nq::solve_on_device(N, rows, cfg, total, stats, nullptr, nullptr);
 }
 
 }  // extern "C"

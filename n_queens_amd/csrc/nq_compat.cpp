@@ -52,28 +52,30 @@ void partial_n_queens_for_odd(int N, int cur, int left, int right, vector<int> &
     append(tot, w);
 }
 
+static void log_generated(void *, uint64_t ntasks, double ms) {
+    nq::log_line("Use %.2fms to generate %lld subproblems!\n", ms, static_cast<long long>(ntasks));
+    nq::log_line("gpu [%d] start job, with %lld(%.2f) subproblems.\n", 0, static_cast<long long>(ntasks), 1.0);
+}
+
 long long cuda_n_queens(int N, int rows) {
-    const double t0 = nq::now_ms();
-    std::vector<uint32_t> tasks;
-    nq::generate_tasks(N, rows, tasks);
-    const uint64_t cnt = tasks.size() / 3;
-    const double t1 = nq::now_ms();
-    nq::log_line("Use %.2fms to generate %lld subproblems!\n", t1 - t0, static_cast<long long>(cnt));
     if (nq_device_count() <= 0) {
+        // The reference generates first and then finds no device (src/n_queens_cuda.cu:91-108).
+        const double t0 = nq::now_ms();
+        std::vector<uint32_t> tasks;
+        nq::generate_tasks(N, rows, tasks);
+        nq::log_line("Use %.2fms to generate %lld subproblems!\n", nq::now_ms() - t0,
+                     static_cast<long long>(tasks.size() / 3));
         printf("Failed to find any gpu!\n");  // src/n_queens_cuda.cu:105-108
         return -1;
     }
-    nq::log_line("gpu [%d] start job, with %lld(%.2f) subproblems.\n", 0, static_cast<long long>(cnt), 1.0);
-    uint64_t sum = 0;
-    // The same device split as nq_solve (totals only; every child keeps its exact count).
-    nq_config solve_cfg;
-    nq_config c{};
-    if (nq_solve_config(N, rows, &solve_cfg) == NQ_OK) c.split = solve_cfg.split;
-    int rc = nq_count(N, tasks.data(), cnt, &c, &sum, nullptr, nullptr, nullptr);
+    // Tasks generated on the device (the same task set and count as the reference's host
+    // generator), split as nq_solve splits them, counted, folded x2.
+    uint64_t total = 0;
+    int rc = nq::solve_on_device(N, rows, nullptr, &total, nullptr, log_generated, nullptr);
     if (rc != NQ_OK) {
         fprintf(stderr, "cuda_n_queens: %s\n", nq_last_error());
         return -1;
     }
     nq::log_line("gpu [%d] finish job.\n", 0);
-    return static_cast<long long>(2 * sum);  // src/n_queens_cuda.cu:181-183
+    return static_cast<long long>(total);
 }

@@ -6,6 +6,8 @@
 #include <string>
 #include <vector>
 
+#include "nq.h"
+
 namespace nq {
 
 // Task list of the reference generators for (N, rows): 3 words per task (nq_prefix.cpp).
@@ -15,6 +17,11 @@ void generate_tasks(int N, int rows, std::vector<uint32_t> &out);
 // odd_part selects partial_n_queens_for_odd's row restrictions.
 void generate_from(int N, bool odd_part, uint32_t cur, uint32_t left, uint32_t right, int rows,
                    std::vector<uint32_t> &out);
+
+// nq_solve with the task list generated on the device (nq_api.cpp); on_generated (nullable)
+// receives the reference's task count for (N, rows) and the generation time in ms.
+int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats,
+                    void (*on_generated)(void *ctx, uint64_t ntasks, double ms), void *ctx);
 
 // Thread-local error reporting for the C ABI.
 int fail(int status, const std::string &msg);

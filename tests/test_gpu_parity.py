@@ -245,3 +245,16 @@ def test_tuner_small_board(tmp_path, published):
     assert tune.tuned_config(13, 3, path=path) == win
     total, _ = nq.solve(13, 3, win)
     assert total == published["solutions"][13]
+
+
+@pytest.mark.parametrize("N,rows", [(8, 3), (9, 4), (12, 2), (13, 5), (14, 7), (15, 1), (16, 6), (11, 9),
+                                    (10, 10), (17, 3)])
+def test_device_generation_matches_host_generator(N, rows):
+    """nq_solve generates the depth-`rows` task list on the device from the depth-2 seeds: the
+    same task count as the reference's host generator, and the same 2 x sum."""
+    tasks = nq.generate(N, rows)
+    total, st = nq.solve(N, rows)
+    assert st["ntasks"] == len(tasks), (N, rows)
+    assert total == 2 * oracle.count_tasks(N, tasks), (N, rows)
+    total0, st0 = nq.solve(N, rows, nq.Config(split=0))
+    assert total0 == total and st0["ntasks"] == len(tasks)
