@@ -212,6 +212,30 @@ int nq_server_timing(nq_server *server, double *device_ms, double *open_ms);
 /* Finish every admitted request, stop the scheduler and free the device queues. */
 int nq_server_close(nq_server *server);
 
+/* ---- resumable counts (no reference counterpart) --------------------------------------
+ * For runs that take days (the reference's N=27 run took 28.4 days with no way to resume,
+ * README.md:46).  The (N, rows) task list is the host generator's (reference order), counted
+ * in consecutive ranges of range_tasks tasks (0 = 1 % of the list) with nq_count.  After every
+ * range the text file `path` is rewritten atomically (written to path.tmp, flushed, renamed)
+ * with N, rows, the task count, the first uncounted task and the partial sum.  A call that finds
+ * the file resumes after its last completed range; a file for another (N, rows) or task count,
+ * or one that does not parse, is NQ_ERR_ARG before any device work.  max_ranges > 0 returns
+ * after that many ranges (state->done = 0 if tasks remain).  cfg as for nq_count; split < 0 or
+ * cfg NULL takes nq_solve's split. */
+typedef struct nq_resume_state {
+    uint64_t ntasks;   /* tasks in the (N, rows) list                                       */
+    uint64_t next;     /* first task not yet counted                                        */
+    uint64_t sum;      /* sum of the per-task counts of tasks [0, next)                     */
+    uint64_t total;    /* 2 * sum once done (symmetry fold), else 0                         */
+    int done;          /* 1 when every task is counted                                       */
+    int ranges;        /* ranges counted by this call                                        */
+    double gen_ms;     /* host generation of the task list                                  */
+    double kernel_ms;  /* device time of this call's ranges                                 */
+} nq_resume_state;
+
+int nq_solve_resumable(int N, int rows, const nq_config *cfg, const char *path, uint64_t range_tasks,
+                       int max_ranges, nq_resume_state *state);
+
 #ifdef __cplusplus
 }
 #endif

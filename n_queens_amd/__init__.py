@@ -39,7 +39,7 @@ EXPORTS = (
     "nq_batch_plan_create", "nq_batch_plan_launch", "nq_batch_plan_wait", "nq_batch_plan_run",
     "nq_batch_plan_destroy", "nq_batch_count", "nq_instance_count",
     "nq_server_config_default", "nq_server_open", "nq_server_submit", "nq_server_poll", "nq_server_wait",
-    "nq_server_stats", "nq_server_timing", "nq_server_close",
+    "nq_server_stats", "nq_server_timing", "nq_server_close", "nq_solve_resumable",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -80,6 +80,12 @@ class _ServerConfig(ctypes.Structure):
 
 class _RequestStats(ctypes.Structure):
     _fields_ = [("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("ntasks", ctypes.c_uint64)]
+
+
+class _ResumeState(ctypes.Structure):
+    _fields_ = [("ntasks", ctypes.c_uint64), ("next", ctypes.c_uint64), ("sum", ctypes.c_uint64),
+                ("total", ctypes.c_uint64), ("done", ctypes.c_int), ("ranges", ctypes.c_int),
+                ("gen_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double)]
 
 
 @dataclass
@@ -139,6 +145,8 @@ def lib() -> ctypes.CDLL:
     L.nq_server_stats.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p]
     L.nq_server_close.argtypes = [ctypes.c_void_p]
     L.nq_server_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.nq_solve_resumable.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Config), ctypes.c_char_p,
+                                     ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_ResumeState)]
     for name in EXPORTS:
         if name not in ("nq_last_error",):
             getattr(L, name).restype = ctypes.c_int
@@ -269,6 +277,18 @@ def batch_count(insts, cfg: Config | None = None) -> tuple[np.ndarray, dict]:
                                 ctypes.byref(st)))
     return out[: len(insts)], {"ntasks": st.ntasks, "total": st.total, "expand_ms": st.expand_ms,
                                "kernel_ms": st.kernel_ms, "total_ms": st.total_ms}
+
+
+def solve_resumable(N: int, rows: int, path: str, range_tasks: int = 0, max_ranges: int = 0,
+                    cfg: Config | None = None) -> dict:
+    """nq_solve_resumable: count (N, rows) in task ranges, checkpointing to `path` after each
+    range and resuming from it.  Returns the state: total (2 x sum) once done."""
+    st = _ResumeState()
+    cc = cfg._c() if cfg else None
+    _check(lib().nq_solve_resumable(N, rows, ctypes.byref(cc) if cc else None, os.fsencode(path), range_tasks,
+                                    max_ranges, ctypes.byref(st)))
+    return {"ntasks": st.ntasks, "next": st.next, "sum": st.sum, "total": st.total, "done": bool(st.done),
+            "ranges": st.ranges, "gen_ms": st.gen_ms, "kernel_ms": st.kernel_ms}
 
 
 def instance_count(N: int, cols, cfg: Config | None = None) -> int:

@@ -1,16 +1,20 @@
-// `n_queens N rows [--config 1|2|3]` — drop-in for the reference CLI (src/main.cpp:8-57).
+// `n_queens N rows [--config 1|2|3] [--checkpoint PATH [--range-tasks K]]` — drop-in for the
+// reference CLI (src/main.cpp:8-57).
 //
 // Same positional arguments, same validation order and messages (src/main.cpp:10-29), same
 // timed call and result line (src/main.cpp:49-54).  The reference picks a (threads, stack
 // depth) preset at compile time (-D_USE_CONFIGn_, src/macro.h:25-35; compile.sh uses CONFIG2);
 // here --config selects the preset at run time and only its stack depth is used, for the
 // legality check.  Missing arguments print a usage line (the reference reads argv unchecked).
+// --checkpoint (no reference counterpart) counts through nq_solve_resumable: the run can be
+// stopped at any time and resumed by running the same command again.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <sys/time.h>
 
 #include "n_queens.h"
+#include "nq.h"
 
 namespace nq {
 void log_line(const char *fmt, ...);
@@ -22,17 +26,23 @@ static float time_diff_ms(const timeval &s, const timeval &e) {
 
 int main(int argc, char *argv[]) {
     int config = 2;
+    const char *checkpoint = nullptr;
+    unsigned long long range_tasks = 0;
     int pos[2];
     int npos = 0;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--config") && i + 1 < argc) {
             config = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--checkpoint") && i + 1 < argc) {
+            checkpoint = argv[++i];
+        } else if (!strcmp(argv[i], "--range-tasks") && i + 1 < argc) {
+            range_tasks = strtoull(argv[++i], nullptr, 10);
         } else if (npos < 2) {
             pos[npos++] = atoi(argv[i]);
         }
     }
     if (npos < 2 || config < 1 || config > 3) {
-        fprintf(stderr, "usage: %s N rows [--config 1|2|3]\n", argv[0]);
+        fprintf(stderr, "usage: %s N rows [--config 1|2|3] [--checkpoint PATH [--range-tasks K]]\n", argv[0]);
         return 2;
     }
     static const int kStack[4] = {0, 24, 19, 16};  // STACKSIZE of CONFIG1/2/3, src/macro.h:25-35
@@ -61,7 +71,19 @@ int main(int argc, char *argv[]) {
     timeval start, end;
     nq::log_line("===============================================================\n");
     gettimeofday(&start, nullptr);
-    long long sum = cuda_n_queens(N, rows);
+    long long sum = -1;
+    if (checkpoint) {
+        nq_resume_state st;
+        if (nq_solve_resumable(N, rows, nullptr, checkpoint, range_tasks, 0, &st) == NQ_OK) {
+            nq::log_line("checkpoint %s: %d ranges counted in this run, tasks %llu of %llu done\n", checkpoint,
+                         st.ranges, static_cast<unsigned long long>(st.next), static_cast<unsigned long long>(st.ntasks));
+            sum = static_cast<long long>(st.total);
+        } else {
+            fprintf(stderr, "n_queens: %s\n", nq_last_error());
+        }
+    } else {
+        sum = cuda_n_queens(N, rows);
+    }
     gettimeofday(&end, nullptr);
     nq::log_line("cuda %d queens result %lld, calc time: [%.2fms]\n", N, sum, time_diff_ms(start, end));
     nq::log_line("===============================================================\n");

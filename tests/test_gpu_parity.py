@@ -258,3 +258,31 @@ def test_device_generation_matches_host_generator(N, rows):
     assert total == 2 * oracle.count_tasks(N, tasks), (N, rows)
     total0, st0 = nq.solve(N, rows, nq.Config(split=0))
     assert total0 == total and st0["ntasks"] == len(tasks)
+
+
+def test_resumable_interrupt_and_resume(tmp_path, published):
+    """Counting in ranges, stopping after 4 of them and resuming from the checkpoint file gives
+    the uninterrupted total; a finished checkpoint returns the total without device work."""
+    p = str(tmp_path / "ck")
+    a = nq.solve_resumable(13, 5, p, range_tasks=1000, max_ranges=4)
+    assert not a["done"] and a["next"] == 4000 and a["ranges"] == 4 and a["total"] == 0
+    tasks = nq.generate(13, 5)
+    assert a["sum"] == oracle.count_tasks(13, tasks[:4000])
+    b = nq.solve_resumable(13, 5, p, range_tasks=3000)
+    assert b["done"] and b["next"] == b["ntasks"] == len(tasks) and b["ranges"] == 4
+    assert b["total"] == published["solutions"][13]
+    c = nq.solve_resumable(13, 5, p)
+    assert c["done"] and c["ranges"] == 0 and c["total"] == b["total"]
+    d = nq.solve_resumable(16, 4, str(tmp_path / "ck16"))
+    assert d["done"] and d["ranges"] == 100 and d["total"] == published["solutions"][16]
+
+
+def test_cli_checkpoint_resume(tmp_path):
+    """`n_queens N rows --checkpoint PATH`: a finished checkpoint is reported by the next run."""
+    p = str(tmp_path / "ck")
+    r = subprocess.run([nq.CLI_PATH, "14", "4", "--checkpoint", p, "--range-tasks", "300"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0 and "cuda 14 queens result 365596, calc time:" in r.stdout, r.stdout + r.stderr
+    assert "ranges counted in this run" in r.stdout
+    r2 = subprocess.run([nq.CLI_PATH, "14", "4", "--checkpoint", p], capture_output=True, text=True, timeout=120)
+    assert "0 ranges counted in this run" in r2.stdout and "cuda 14 queens result 365596" in r2.stdout

@@ -4,6 +4,7 @@ drop-in links, and the CLI reproduces the reference's argument validation."""
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -123,3 +124,46 @@ def test_tuner_candidates_respect_depth():
         assert all(rows + c.split + 2 < N or c.split == 0 for c in cands)
         assert {c.push for c in cands} == {nq.NQ_PUSH_MASKED, nq.NQ_PUSH_ALWAYS}
     assert len(tune.candidates(21, 6, quick=True)) == 8
+
+
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def test_resumable_rejects_foreign_checkpoints(tmp_path):
+    """A checkpoint for another problem, another task count, or not a checkpoint at all is
+    refused before any device work (so this runs without a GPU)."""
+    p = str(tmp_path / "ck")
+    _write(p, "nq-checkpoint 1\nN 13 rows 5 ntasks 15550 next 0 sum 0\n")
+    with pytest.raises(nq.NQError) as e:
+        nq.solve_resumable(14, 5, p)
+    assert e.value.status == nq.NQ_ERR_ARG and "different problem" in str(e.value)
+    _write(p, "nq-checkpoint 1\nN 13 rows 5 ntasks 999 next 0 sum 0\n")
+    with pytest.raises(nq.NQError) as e:
+        nq.solve_resumable(13, 5, p)
+    assert "task count" in str(e.value)
+    _write(p, "nq-checkpoint 1\nN 13 rows 5 ntasks 15550 next 20000 sum 0\n")
+    with pytest.raises(nq.NQError) as e:
+        nq.solve_resumable(13, 5, p)
+    assert "past its task count" in str(e.value)
+    _write(p, "hello\n")
+    with pytest.raises(nq.NQError) as e:
+        nq.solve_resumable(13, 5, p)
+    assert "not a checkpoint" in str(e.value)
+
+
+def test_resumable_writes_initial_checkpoint_without_gpu(tmp_path):
+    """With no device the call fails with NQ_ERR_NO_DEVICE after writing the initial checkpoint
+    (cursor 0, the generator's task count)."""
+    p = str(tmp_path / "ck")
+    r = subprocess.run([sys.executable, "-c", f"""
+import n_queens_amd as nq
+try:
+    nq.solve_resumable(13, 5, {p!r}, range_tasks=1000)
+except nq.NQError as e:
+    print("status", e.status)
+"""], capture_output=True, text=True, env=NO_GPU, cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+        timeout=120)
+    assert f"status {nq.NQ_ERR_NO_DEVICE}" in r.stdout, r.stdout + r.stderr
+    assert open(p).read() == "nq-checkpoint 1\nN 13 rows 5 ntasks 15550 next 0 sum 0\n"
