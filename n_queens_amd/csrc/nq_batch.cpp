@@ -107,8 +107,14 @@ bool encode_instance(const nq_instance &in, uint32_t *c, uint32_t *l, uint32_t *
     return true;
 }
 
-// Validate instances and build the device task list (largest expected subtrees first).
-int build_batch_tasks(const nq_instance *inst, int n, int depth_target, std::vector<uint4> &tasks, int *max_frames) {
+// Validate instances and build the seeds of the device task list, largest expected subtrees
+// first: each instance's prefix, or for an empty board (N >= 4) the reference's
+// mirror-symmetric tasks at depth 2 (weight 2).  batch_expand then expands every seed to
+// `depth_target` free rows on the device, in the order append_instance_tasks would produce on
+// the host.  *max_frames is the deepest DFS stack any final task needs, *levels the most rows
+// any seed is expanded by.
+int build_batch_seeds(const nq_instance *inst, int n, int depth_target, std::vector<uint4> &seeds, int *max_frames,
+                      int *levels) {
     std::vector<Prefix> pre(n);
     for (int i = 0; i < n; ++i) {
         if (!encode(inst[i], &pre[i])) {
@@ -129,15 +135,27 @@ int build_batch_tasks(const nq_instance *inst, int n, int depth_target, std::vec
         const int ra = rem_rows(a), rb = rem_rows(b);
         return ra != rb ? ra > rb : inst[a].N > inst[b].N;
     });
-    tasks.clear();
-    int frames = 1;
+    seeds.clear();
+    int frames = 1, lev = 0;
     for (int i : order) {
         const int N = inst[i].N, k = inst[i].k;
-        const int d0 = append_instance_tasks(N, k, pre[i].c, pre[i].l, pre[i].r, depth_target, static_cast<uint32_t>(i),
-                                             tasks);
+        const int s = split_rows(N, k, depth_target);
+        int seed_depth = k, d0 = k + s;
+        if (k == 0 && N >= 4) {
+            std::vector<uint32_t> t;
+            generate_tasks(N, 2, t);
+            const uint32_t tag = (static_cast<uint32_t>(i) << 6) | (1u << 5) | static_cast<uint32_t>(N);
+            for (size_t j = 0; j + 2 < t.size(); j += 3) seeds.push_back({t[j], t[j + 1], t[j + 2], tag});
+            seed_depth = 2;
+            d0 = std::max(2, s);
+        } else {
+            seeds.push_back({pre[i].c, pre[i].l, pre[i].r, (static_cast<uint32_t>(i) << 6) | static_cast<uint32_t>(N)});
+        }
+        lev = std::max(lev, d0 - seed_depth);
         if (d0 + 1 < N) frames = std::max(frames, N - 1 - d0);
     }
     *max_frames = frames;
+    *levels = lev;
     return NQ_OK;
 }
 
@@ -216,9 +234,9 @@ int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, n
     if (c.chunk < 1 || c.chunk > 4096) return fail(NQ_ERR_ARG, "chunk must be 1..4096");
     // Per-task depth target: the stack depth + 1 when given, else 12 free rows (11 frames).
     const int depth_target = c.stack_depth ? c.stack_depth + 1 : 12;
-    std::vector<uint4> tasks;
-    int frames = 1;
-    int rc = build_batch_tasks(inst, n, depth_target, tasks, &frames);
+    std::vector<uint4> tasks;  // seeds; expanded on the device below
+    int frames = 1, levels = 0;
+    int rc = build_batch_seeds(inst, n, depth_target, tasks, &frames, &levels);
     if (rc) return rc;
     if (!c.stack_depth) c.stack_depth = frames;
     if (c.stack_depth < frames || c.stack_depth > 32) return fail(NQ_ERR_DEPTH, "stack depth out of range for the batch");
@@ -228,7 +246,6 @@ int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, n
     auto *p = new nq_batch_plan();
     p->ninst = static_cast<uint64_t>(n);
     p->ntasks = tasks.size();
-    p->expand_ms = now_ms() - t0;
     hipError_t e = hipGetDevice(&p->device);
     int cus = 0;
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device);
@@ -241,10 +258,12 @@ int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, n
     if (e == hipSuccess && !tasks.empty())
         e = hipMemcpyAsync(p->d_tasks, tasks.data(), tasks.size() * sizeof(uint4), hipMemcpyHostToDevice, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e == hipSuccess) e = batch_expand(&p->d_tasks, &p->ntasks, static_cast<uint32_t>(depth_target), levels, p->stream);
     if (e != hipSuccess) {
         plan_free(p);
-        return fail_hip(e, "batch plan allocation/upload");
+        return fail_hip(e, "batch plan allocation/upload/expansion");
     }
+    p->expand_ms = now_ms() - t0;
     bool apush = false;
     int bpc = 0;
     e = batch_fit(c.block, c.stack_depth, c.waves_per_cu, c.push, &apush, &bpc);

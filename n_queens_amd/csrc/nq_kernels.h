@@ -48,6 +48,12 @@ struct BatchArgs {
     bool apush = false;  // unconditional push (stack_depth + 1 levels)
 };
 
+// Expand a batched task list (uint4 {cur, left, right, tag}, N in tag bits 0..4) on the device,
+// up to `levels` rows, until no task has more than `target` free rows, keeping the order a
+// depth-first expansion of each task gives.  *tasks (hipMalloc'd) is replaced by a new buffer
+// and *n by its count; synchronizes the stream (nq_expand.hip).
+hipError_t batch_expand(uint4 **tasks, uint64_t *n, uint32_t target, int levels, hipStream_t stream);
+
 // Enqueue: zero counts, reset state, batched DFS.  Capturable into a hipGraph (kernels only).
 hipError_t launch_batch(const BatchArgs &a, hipStream_t stream);
 hipError_t batch_occupancy(int block, int stack_depth, bool apush, int *blocks_per_cu);
