@@ -5,7 +5,7 @@
 // is counted) and src/n_queens_cuda.cu:9-82 (the GPU kernel it replaces).
 //
 // Design (one lane = one independent DFS, 64 lanes per wave):
-//  * The frame being expanded lives in VGPRs: c (columns + depth), l, r (diagonals), v
+//  * The frame being expanded lives in VGPRs: c (occupied columns), l, r (diagonals), v
 //    (candidates not yet tried).  One loop iteration places exactly one queen (one DFS node)
 //    on every active lane; there is no per-iteration reload of the top frame.
 //  * Before a frame is replaced by its child it is pushed to LDS only if it still has
@@ -222,11 +222,11 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
     uint32_t sp = base;
 
     uint32_t c = 0, l = 0, r = 0, v = 0;  // the frame being expanded
-    uint32_t cnt = 0;     // last-row hits since the last fold (at most one per iteration)
+    uint32_t cnt = 0;     // completed boards since the last fold (at most one per iteration)
     uint32_t nd = 0;      // expansions since the last fold (node-counting variant)
     uint64_t tsum = 0;    // folded count of the current task
     uint64_t acc = 0;     // counts of finished tasks
-    uint64_t nodes = 0;   // expansions minus last-row hits, i.e. DFS nodes
+    uint64_t nodes = 0;   // expansions minus completed boards, i.e. DFS nodes
     uint64_t tid = ~0ull;
     bool has = true;      // lane still has (or may claim) work
 

@@ -167,3 +167,12 @@ except nq.NQError as e:
         timeout=120)
     assert f"status {nq.NQ_ERR_NO_DEVICE}" in r.stdout, r.stdout + r.stderr
     assert open(p).read() == "nq-checkpoint 1\nN 13 rows 5 ntasks 15550 next 0 sum 0\n"
+
+
+def test_tuner_occupancy_candidates():
+    from n_queens_amd import tune
+    win = nq.Config(block=128, chunk=16, split=2)
+    occ = tune.occupancy_candidates(20, 6, win)
+    assert [c.waves_per_cu for c in occ[:4]] == [4, 8, 12, 16]
+    assert [c.stack_depth for c in occ[4:]] == [13, 15, 17]
+    assert all(c.split == 2 and c.block == 128 for c in occ)
