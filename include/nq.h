@@ -26,7 +26,7 @@ enum nq_status {
     NQ_ERR_NO_DEVICE = -2,/* no HIP device visible (reference: n_queens_cuda.cu:105-108)   */
     NQ_ERR_HIP = -3,      /* a HIP runtime call failed (reference: CU_SAFE_CALL, macro.h)  */
     NQ_ERR_DEPTH = -4,    /* stack depth too small for (N, prefix depth)                   */
-    NQ_ERR_NOMEM = -5,    /* host or device allocation failed                              */
+    NQ_ERR_NOMEM = -5,    /* allocation failed, or a request larger than a server slot     */
     NQ_ERR_BUSY = -6,     /* admission refused (serving: task or LDS budget exhausted)     */
     NQ_ERR_STATE = -7     /* call not valid in the handle's current state                  */
 };
@@ -170,7 +170,8 @@ int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *c
  * A server keeps `slots` device-resident task queues (tasks, per-request counters, cursor,
  * pinned result buffer, stream).  nq_server_submit validates a request, expands it into tasks
  * on the caller's thread and admits it while the pending work fits max_pending_tasks (else
- * NQ_ERR_BUSY; NQ_ERR_BUSY also for one request larger than slot_tasks).  A request is
+ * NQ_ERR_BUSY, which a later retry may pass; a request larger than slot_tasks can never fit
+ * and is refused with NQ_ERR_NOMEM).  A request is
  * expanded until no task keeps more than depth_target free rows.  A scheduler thread
  * cuts a batch into each free slot: pending requests by least estimated remaining work (aged
  * by waiting time), each contributing tasks until the batch's estimated work (batch_work DFS
@@ -180,14 +181,14 @@ int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *c
  * thread-safe. */
 typedef struct nq_server_config {
     int slots;                   /* concurrent batches in flight (default 3)               */
-    uint64_t slot_tasks;         /* task capacity of one slot (default 4M)                 */
+    uint64_t slot_tasks;         /* task capacity of one slot (default 16M)                */
     int slot_requests;           /* requests per batch (default 4096)                      */
-    uint64_t max_pending_tasks;  /* admission budget of queued tasks (default 16M)         */
-    int depth_target;            /* most free rows a task keeps (default 13): bounds both  */
+    uint64_t max_pending_tasks;  /* admission budget of queued tasks (default 64M)         */
+    int depth_target;            /* most free rows a task keeps (default 12): bounds both  */
                                  /* the LDS stack and one task's run time on its lane      */
     int block;                   /* workgroup size (default 128)                           */
-    int chunk;                   /* tasks per cursor grab (default 4)                      */
-    double batch_work;           /* estimated DFS nodes per batch (default 2e10, ~20 ms)   */
+    int chunk;                   /* tasks per cursor grab (default 16)                     */
+    double batch_work;           /* estimated DFS nodes per batch (default 1e10, ~5 ms)    */
 } nq_server_config;
 
 typedef struct nq_request_stats {

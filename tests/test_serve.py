@@ -81,8 +81,8 @@ def test_admission_budget_and_invalid_requests():
             srv.submit(10, [0, 1])
         assert e.value.status == nq.NQ_ERR_ARG
         with pytest.raises(nq.NQError) as e:
-            srv.submit(19, [])          # expands to more tasks than one slot holds
-        assert e.value.status == nq.NQ_ERR_BUSY
+            srv.submit(19, [])          # expands to more tasks than one slot holds: never fits
+        assert e.value.status == nq.NQ_ERR_NOMEM
         busy = 0
         tickets = []
         for _ in range(200):
