@@ -242,6 +242,14 @@ def main() -> int:
     if int(res[2]):
         log("ERROR: device error flags set")
         return 1
+    with open(os.path.join(ROOT, "tests", "golden", "published.json")) as f:
+        fixtures = json.load(f)
+    want_nodes = fixtures.get("dfs_nodes", {}).get(f"{N},{rows}")
+    want_total = fixtures["solutions"].get(str(N))
+    if (want_nodes is not None and nodes != want_nodes) or (rows >= 2 and want_total is not None
+                                                             and 2 * sum0 != want_total):
+        log(f"ERROR: nodes {nodes} / total {2 * sum0} differ from the fixtures {want_nodes} / {want_total}")
+        return 1
     for i in range(max(0, args.warmup - 1)):
         d_result.zero_()
         run(False)

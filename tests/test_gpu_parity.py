@@ -57,7 +57,7 @@ def test_p2_rows5_vs_published_and_oracle_sample(N, published):
     assert np.array_equal(r.per_task, expect)
 
 
-@pytest.mark.parametrize("N,rows", [(12, 3), (13, 5), (14, 4)])
+@pytest.mark.parametrize("N,rows", [(12, 3), (13, 5), (14, 4), (16, 4), (17, 5)])
 def test_node_count_vs_oracle(N, rows):
     t = nq.generate(N, rows)
     r = nq.count(N, t, nodes=True)
@@ -286,3 +286,21 @@ def test_cli_checkpoint_resume(tmp_path):
     assert "ranges counted in this run" in r.stdout
     r2 = subprocess.run([nq.CLI_PATH, "14", "4", "--checkpoint", p], capture_output=True, text=True, timeout=120)
     assert "0 ranges counted in this run" in r2.stdout and "cuda 14 queens result 365596" in r2.stdout
+
+
+def test_node_count_n18_vs_oracle_threads():
+    """N=18 rows 6 (2.9e10 nodes): the GPU node kernel against the oracle's threaded counter."""
+    N, rows = 18, 6
+    t = nq.generate(N, rows)
+    r = nq.count(N, t, nodes=True)
+    assert r.nodes == oracle.count_tasks_nodes(N, t, threads=16)
+    assert 2 * r.sum == 666090624
+
+
+def test_n21_node_count_fixture(published):
+    """The N=21 rows 6 node count that bench.py's nodes/s is computed from."""
+    N, rows = 21, 6
+    t = nq.generate(N, rows)
+    r = nq.count(N, t, nodes=True)
+    assert r.nodes == published["dfs_nodes"]["21,6"]
+    assert 2 * r.sum == published["solutions"][N]
