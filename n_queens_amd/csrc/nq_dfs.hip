@@ -419,58 +419,85 @@ __global__ void fold_result_kernel(const DevState *st, uint64_t *result, const u
     }
 }
 
+constexpr uint32_t EXPAND_BLOCK = 256;
+constexpr uint32_t EXPAND_ITEMS = 8;  // tasks per thread per tile
+
 // Expand every task by one row: out gets (cur|p, (left|p)<<1, (right|p)>>1) for every legal
 // queen p of the next row, and a full board is carried over as it is; the counts of a task's
-// outputs sum to its own count.  Outputs are placed through one wave-aggregated atomic on
-// *out_n, so their order is not the input's: only totals use this.
-__global__ void expand_kernel(const uint32_t *__restrict__ in, uint64_t n, const unsigned long long *__restrict__ n_dev,
-                              uint32_t N, uint32_t *__restrict__ out, uint64_t cap,
-                              unsigned long long *__restrict__ out_n, unsigned long long *__restrict__ flags) {
+// outputs sum to its own count.  The order is not the input's: only totals use this.  A block
+// works on tiles of EXPAND_BLOCK * EXPAND_ITEMS tasks and reserves a tile's outputs with one
+// atomic on *out_n (one per task wave made that counter the bottleneck, ~1e8 atomics/s on one
+// address); waves then write their outputs round by round, every lane with a j-th output
+// writing it to the next consecutive slot, so one store instruction covers a contiguous range.
+__global__ void __launch_bounds__(EXPAND_BLOCK)
+expand_kernel(const uint32_t *__restrict__ in, uint64_t n, const unsigned long long *__restrict__ n_dev,
+              uint32_t N, uint32_t *__restrict__ out, uint64_t cap, unsigned long long *__restrict__ out_n,
+              unsigned long long *__restrict__ flags) {
     if (n_dev) n = min(n, static_cast<uint64_t>(*n_dev));
+    __shared__ unsigned long long wave_out[EXPAND_BLOCK / 64];
+    __shared__ unsigned long long tile_base;
     const uint32_t last = (1u << N) - 1u;
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
-        const uint64_t i = base + threadIdx.x;
-        uint32_t c = 0, l = 0, r = 0, v = 0;
-        if (i < n) {
-            c = in[3 * i];
-            l = in[3 * i + 1];
-            r = in[3 * i + 2];
-            v = last & ~(c | l | r);
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t tile = static_cast<uint64_t>(EXPAND_BLOCK) * EXPAND_ITEMS;
+    for (uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * tile; t0 < n; t0 += static_cast<uint64_t>(gridDim.x) * tile) {
+        // This thread's tasks: t0 + j * EXPAND_BLOCK + threadIdx.x (coalesced loads).
+        uint32_t c[EXPAND_ITEMS], l[EXPAND_ITEMS], r[EXPAND_ITEMS], v[EXPAND_ITEMS];
+        uint32_t k = 0;
+        for (uint32_t j = 0; j < EXPAND_ITEMS; ++j) {
+            const uint64_t i = t0 + static_cast<uint64_t>(j) * EXPAND_BLOCK + threadIdx.x;
+            c[j] = l[j] = r[j] = v[j] = 0;
+            if (i < n) {
+                c[j] = in[3 * i];
+                l[j] = in[3 * i + 1];
+                r[j] = in[3 * i + 2];
+                // A full board has no children but counts 1: carried over as its own output,
+                // marked by v == ~0 (a real candidate set never has bits above N).
+                v[j] = c[j] == last ? ~0u : (last & ~(c[j] | l[j] | r[j]));
+            }
+            k += v[j] == ~0u ? 1u : __popc(v[j]);
         }
-        // A full board has no children but counts 1: carry it over unchanged.
-        const bool full = (i < n) && c == last;
-        // Wave-inclusive prefix sum of the child counts, one atomic per wave.
-        const uint32_t k = full ? 1u : __popc(v);
-        uint32_t incl = k;
-        const uint32_t lane = threadIdx.x & 63u;
-        for (uint32_t off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += y;
+        const uint64_t wk = wave_sum_u64(k);
+        if ((threadIdx.x & 63u) == 0) wave_out[wave] = wk;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long total = 0;
+            for (uint32_t w = 0; w < EXPAND_BLOCK / 64; ++w) total += wave_out[w];
+            tile_base = total ? atomicAdd(out_n, total) : 0;
+            if (tile_base + total > cap) {
+                // More outputs than the buffer holds (the caller's size was wrong): write
+                // nothing past it and report; the consumers clamp their counts to `cap`.
+                atomicOr(flags, 2ull);
+                tile_base = ~0ull;
+            }
         }
-        const uint32_t wave_total = __shfl(incl, 63, 64);
-        unsigned long long wbase = 0;
-        if (lane == 63 && wave_total) wbase = atomicAdd(out_n, static_cast<unsigned long long>(wave_total));
-        wbase = __shfl(wbase, 63, 64);
-        uint64_t o = wbase + (incl - k);
-        if (o + k > cap) {
-            // More outputs than the buffer holds (the caller's depth bound was wrong): write
-            // nothing past it and report; the consumers clamp their counts to `cap`.
-            if (k) atomicOr(flags, 2ull);
-            continue;
-        }
-        if (full) {
-            out[3 * o] = c;
-            out[3 * o + 1] = l;
-            out[3 * o + 2] = r;
-        }
-        while (v) {
-            const uint32_t p = v & (0u - v);
-            v ^= p;
-            out[3 * o] = c | p;
-            out[3 * o + 1] = (l | p) << 1;
-            out[3 * o + 2] = (r | p) >> 1;
-            ++o;
+        __syncthreads();
+        const unsigned long long tb = tile_base;
+        unsigned long long o = tb;
+        for (uint32_t w = 0; w < wave; ++w) o += wave_out[w];
+        __syncthreads();  // wave_out and tile_base are rewritten by the next tile
+        if (tb == ~0ull) continue;
+        for (uint32_t j = 0; j < EXPAND_ITEMS; ++j) {
+            uint32_t vj = v[j];
+            for (;;) {
+                const uint64_t m = __ballot(vj != 0);
+                if (m == 0) break;
+                if (vj != 0) {
+                    const uint64_t slot = o + lane_rank(m);
+                    if (vj == ~0u) {
+                        out[3 * slot] = c[j];
+                        out[3 * slot + 1] = l[j];
+                        out[3 * slot + 2] = r[j];
+                        vj = 0;
+                    } else {
+                        const uint32_t p = vj & (0u - vj);
+                        vj ^= p;
+                        out[3 * slot] = c[j] | p;
+                        out[3 * slot + 1] = (l[j] | p) << 1;
+                        out[3 * slot + 2] = (r[j] | p) >> 1;
+                    }
+                }
+                o += __popcll(m);
+            }
         }
     }
 }
@@ -580,8 +607,9 @@ hipError_t launch_expand(const uint32_t *in, uint64_t n_max, const unsigned long
     zero_u64_kernel<<<1, 64, 0, stream>>>(out_n, 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const uint64_t blocks = std::min<uint64_t>((n_max + 255) / 256, 16384);
-    expand_kernel<<<static_cast<unsigned>(std::max<uint64_t>(1, blocks)), 256, 0, stream>>>(
+    const uint64_t tile = static_cast<uint64_t>(EXPAND_BLOCK) * EXPAND_ITEMS;
+    const uint64_t blocks = std::min<uint64_t>((n_max + tile - 1) / tile, 8192);
+    expand_kernel<<<static_cast<unsigned>(std::max<uint64_t>(1, blocks)), EXPAND_BLOCK, 0, stream>>>(
         in, n_max, n_dev, static_cast<uint32_t>(N), out, cap, out_n, flags);
     return hipGetLastError();
 }
