@@ -231,7 +231,7 @@ def main() -> int:
                         cfg=c or (cfg_nodes if count_nodes else cfg), count_nodes=count_nodes,
                         stream=stream.cuda_stream)
 
-    # Node count (node-counting kernel, once; also the first warmup).
+    # Node count (node-counting kernel, once, untimed; not a warmup of the timed path).
     d_result.zero_()
     torch.cuda.synchronize()
     run(True)
@@ -250,7 +250,8 @@ def main() -> int:
                                                              and 2 * sum0 != want_total):
         log(f"ERROR: nodes {nodes} / total {2 * sum0} differ from the fixtures {want_nodes} / {want_total}")
         return 1
-    for i in range(max(0, args.warmup - 1)):
+    # Warmup: W untimed steps of exactly the timed work (they also allocate the split buffers).
+    for i in range(args.warmup):
         d_result.zero_()
         run(False)
         stream.synchronize()
