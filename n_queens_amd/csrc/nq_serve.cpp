@@ -148,7 +148,15 @@ static void plan_slot(nq_server *S, Slot &s, double now) {
 static hipError_t launch_slot(nq_server *S, Slot &s) {
     uint64_t n = 0;
     int frames = 1;
-    for (size_t i = 0; i < s.slices.size(); ++i) {
+    // Pack the slices with the largest tasks first: the batch ends when its last task does, so
+    // the longest tasks should be claimed first and the short ones fill the tail.  Counters are
+    // per slice (the tag's index), so the order changes no count.
+    std::vector<size_t> order(s.slices.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return s.slices[a].req->task_work > s.slices[b].req->task_work;
+    });
+    for (size_t i : order) {
         const Slice &sl = s.slices[i];
         const uint32_t idx = static_cast<uint32_t>(i);
         for (size_t j = sl.begin; j < sl.end; ++j) {
