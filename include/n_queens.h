@@ -1,17 +1,20 @@
-// n_queens.h — C++ drop-in for the reference's GPU counting path.
+// n_queens.h — C++ drop-in for the reference's counting entry points.
 //
-// Same names, signatures and C++ linkage as the reference header src/n_queens.h:7-19 for the
-// functions on the GPU path, so that a main.cpp-style caller (src/main.cpp:51) links unchanged
-// against libnq_hip.so.  cuda_n_queens keeps its name as "the GPU count" even though it runs
-// HIP on an MI355X; it is implemented on top of the C ABI in nq.h.
-//
-// The reference's CPU-only counters (n_queens, n_queens_iterative, serial_n_queens,
-// parallel_n_queens; src/n_queens.h:7-17) are not part of this GPU library.
+// Same names, signatures and C++ linkage as the reference header src/n_queens.h:7-19, so that a
+// main.cpp-style caller (src/main.cpp:51, and the CPU cross-checks commented out at :35-47)
+// links unchanged against libnq_hip.so.  cuda_n_queens keeps its name as "the GPU count" even
+// though it runs HIP on an MI355X; it is implemented on top of the C ABI in nq.h.  The CPU
+// counters run on the host only and are never used by the GPU path.  n_queens_iterative, an
+// empty stub in the reference (src/n_queens.cpp:24-26), is not provided.
 #pragma once
 
 #include <vector>
 
 using namespace std;  // as the reference header does (src/n_queens.h:5)
+
+// n_queens (src/n_queens.cpp:8-21): add the number of completions of the prefix
+// (cur, left, right) to sum (which the caller initialises).
+void n_queens(int N, int cur, int left, int right, long long &sum);
 
 // partial_n_queens (src/n_queens.cpp:74-91): append the (cur, left, right) triple of every
 // placement of `rows` more queens below (cur, left, right) to tot; row 0 (cur == 0) is limited
@@ -27,3 +30,8 @@ void partial_n_queens_for_odd(int N, int cur, int left, int right, vector<int> &
 // when no GPU is present (src/n_queens_cuda.cu:105-108) and, unlike the reference (which calls
 // exit(-1), src/macro.h:40-49), also -1 after any other failure, with the reason on stderr.
 long long cuda_n_queens(int N, int rows);
+
+// serial_n_queens / parallel_n_queens (src/n_queens.cpp:115-163): the same total on the host,
+// task by task over the generator's list, on one thread or on 32 threads.
+long long serial_n_queens(int N, int rows);
+long long parallel_n_queens(int N, int rows);

@@ -7,7 +7,9 @@
 // here --config selects the preset at run time and only its stack depth is used, for the
 // legality check.  Missing arguments print a usage line (the reference reads argv unchecked).
 // --checkpoint (no reference counterpart) counts through nq_solve_resumable: the run can be
-// stopped at any time and resumed by running the same command again.
+// stopped at any time and resumed by running the same command again.  --serial and --openmp
+// first run the reference's host cross-checks (commented out at src/main.cpp:35-47), printing
+// its "serial ..." / "parallel ..." lines.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -27,6 +29,7 @@ static float time_diff_ms(const timeval &s, const timeval &e) {
 int main(int argc, char *argv[]) {
     int config = 2;
     const char *checkpoint = nullptr;
+    bool serial = false, openmp = false;
     unsigned long long range_tasks = 0;
     int pos[2];
     int npos = 0;
@@ -35,6 +38,10 @@ int main(int argc, char *argv[]) {
             config = atoi(argv[++i]);
         } else if (!strcmp(argv[i], "--checkpoint") && i + 1 < argc) {
             checkpoint = argv[++i];
+        } else if (!strcmp(argv[i], "--serial")) {
+            serial = true;
+        } else if (!strcmp(argv[i], "--openmp")) {
+            openmp = true;
         } else if (!strcmp(argv[i], "--range-tasks") && i + 1 < argc) {
             range_tasks = strtoull(argv[++i], nullptr, 10);
         } else if (npos < 2) {
@@ -42,7 +49,8 @@ int main(int argc, char *argv[]) {
         }
     }
     if (npos < 2 || config < 1 || config > 3) {
-        fprintf(stderr, "usage: %s N rows [--config 1|2|3] [--checkpoint PATH [--range-tasks K]]\n", argv[0]);
+        fprintf(stderr, "usage: %s N rows [--config 1|2|3] [--serial] [--openmp] [--checkpoint PATH [--range-tasks K]]\n",
+                argv[0]);
         return 2;
     }
     static const int kStack[4] = {0, 24, 19, 16};  // STACKSIZE of CONFIG1/2/3, src/macro.h:25-35
@@ -69,6 +77,19 @@ int main(int argc, char *argv[]) {
     }
 
     timeval start, end;
+    long long cpu_sum = 0;
+    if (serial) {
+        gettimeofday(&start, nullptr);
+        cpu_sum = serial_n_queens(N, rows);
+        gettimeofday(&end, nullptr);
+        printf("serial %d queens result %lld, calc time: [%.2fms]\n", N, cpu_sum, time_diff_ms(start, end));
+    }
+    if (openmp) {
+        gettimeofday(&start, nullptr);
+        cpu_sum = parallel_n_queens(N, rows);
+        gettimeofday(&end, nullptr);
+        printf("parallel %d queens result %lld, calc time: [%.2fms]\n", N, cpu_sum, time_diff_ms(start, end));
+    }
     nq::log_line("===============================================================\n");
     gettimeofday(&start, nullptr);
     long long sum = -1;

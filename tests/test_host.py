@@ -82,6 +82,29 @@ def test_cpp_dropin_links_and_generates(tmp_path):
     assert np.array_equal(np.array(words[1:], dtype=np.int64).astype(np.uint32).reshape(-1, 3), expect)
 
 
+def test_cpp_dropin_cpu_counters(tmp_path, published):
+    """The reference header's host counters (src/n_queens.h:7-17): n_queens accumulates into the
+    caller's sum; serial_n_queens and parallel_n_queens fold x2 over the generator's tasks."""
+    src = tmp_path / "cpu.cpp"
+    src.write_text(
+        '#include "n_queens.h"\n#include <cstdio>\n'
+        "int main(){ long long s = 5; n_queens(8, 0, 0, 0, s);\n"
+        " vector<int> t; partial_n_queens(11,0,0,0,t,3); partial_n_queens_for_odd(11,0,0,0,t,3); long long p = 0;\n"
+        " for (size_t i = 0; i < t.size(); i += 3) n_queens(11, t[i], t[i+1], t[i+2], p);\n"
+        ' printf("%lld %lld", s, p);\n'
+        " for (int N = 4; N <= 13; ++N) printf(\" %lld %lld\", serial_n_queens(N, 2), parallel_n_queens(N, N < 8 ? 2 : 4));\n"
+        ' printf("\\n"); return 0; }\n')
+    exe = tmp_path / "cpu"
+    libdir = os.path.dirname(nq.LIB_PATH)
+    subprocess.check_call(["g++", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                           "-L", libdir, "-lnq_hip", f"-Wl,-rpath,{libdir}"])
+    w = [int(x) for x in subprocess.check_output([str(exe)], env=NO_GPU).split()]
+    assert w[0] == 5 + 92                                   # accumulates into sum
+    assert w[1] == oracle.count_tasks(11, oracle.gen_tasks(11, 3))
+    for i, N in enumerate(range(4, 14)):
+        assert w[2 + 2 * i] == w[3 + 2 * i] == published["solutions"][N], N
+
+
 NO_GPU = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
 
 
@@ -103,6 +126,16 @@ def test_cli_validation_order():
     assert "This configure" not in p.stdout
     p = run_cli(13, 12)
     assert p.stdout.strip() == "Pre-placed rows should be <= 11."
+
+
+def test_cli_cpu_cross_checks():
+    """--serial / --openmp print the reference's commented-out cross-check lines
+    (src/main.cpp:35-47) before the GPU part."""
+    p = run_cli(12, 4, "--serial", "--openmp")
+    lines = p.stdout.splitlines()
+    assert re.match(r"serial 12 queens result 14200, calc time: \[[\d.]+ms\]$", lines[0])
+    assert re.match(r"parallel 12 queens result 14200, calc time: \[[\d.]+ms\]$", lines[1])
+    assert "Failed to find any gpu!" in p.stdout
 
 
 def test_cli_no_gpu_path():
