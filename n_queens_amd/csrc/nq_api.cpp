@@ -371,6 +371,7 @@ int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_s
     }
     NQ_HIP(hipStreamSynchronize(d->stream));
     if (res[2] & 2) return fail(NQ_ERR_STATE, "device expansion overflowed its buffer");
+    if ((st.error | res[2]) & DEV_ERR_LDS) return fail(NQ_ERR_STATE, "DFS kernel LDS does not start at address 0");
     if (st.error || (res[2] & 1)) return fail(NQ_ERR_DEPTH, "a task needed more stack frames than configured");
     *total = 2 * res[0];  // symmetry fold, src/n_queens_cuda.cu:181-183
     if (stats) {
@@ -556,6 +557,7 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     }
     NQ_HIP(hipStreamSynchronize(d->stream));
     if (res[2] & 2) return fail(NQ_ERR_STATE, "device expansion overflowed its buffer");
+    if ((st.error | res[2]) & DEV_ERR_LDS) return fail(NQ_ERR_STATE, "DFS kernel LDS does not start at address 0");
     if (st.error || (res[2] & 1)) return fail(NQ_ERR_DEPTH, "a task needed more stack frames than configured");
     *sum = res[0];
     if (nodes) *nodes = res[1];

@@ -311,6 +311,7 @@ int nq_batch_plan_wait(nq_batch_plan *p, uint64_t *counts, nq_batch_stats *stats
         NQ_HIP(hipMemcpyAsync(counts, p->d_counts, p->ninst * sizeof(uint64_t), hipMemcpyDeviceToHost, p->stream));
     NQ_HIP(hipMemcpyAsync(&st, p->d_state, sizeof st, hipMemcpyDeviceToHost, p->stream));
     NQ_HIP(hipStreamSynchronize(p->stream));
+    if (st.error & DEV_ERR_LDS) return fail(NQ_ERR_STATE, "batch kernel LDS does not start at address 0");
     if (st.error) return fail(NQ_ERR_DEPTH, "a batched task needed more stack frames than configured");
     if (stats) {
         float ms = 0;

@@ -14,9 +14,11 @@
 //    consecutive 16-B slots, so ds_write_b128 / ds_read_b128 are bank-conflict free whatever
 //    depth each lane is at.
 //  * The columns word c has every bit from N up set: the candidates of a frame are then
-//    ~(l | r | c) (one v_bitop3, no board mask) and a completed board is c == ~0, counted with
-//    a compare and an add-with-carry.  Nothing in the iteration depends on N, so lanes of one
-//    wave may work on different board sizes (batch_kernel).
+//    ~(l | r | c) (one v_bitop3, no board mask) and a completed board is c == ~0.  It is
+//    counted with a compare and either an add-with-carry into a per-lane counter (per-task
+//    counts, node counts, batches) or a scalar popcount of the compare mask into a per-wave
+//    counter (plain totals: one VALU fewer per node).  Nothing in the iteration depends on N,
+//    so lanes of one wave may work on different board sizes (batch_kernel).
 //  * Two push forms: exec-masked (only frames with candidates left are written), or an
 //    unconditional write with sp advanced by stride * min(v, 1) (no exec switch, one spare
 //    stack level); the host takes the second when the spare level costs no occupancy.
@@ -40,6 +42,8 @@ namespace {
 // Hot-loop trips (8 iterations each) between folds of the 32-bit per-lane counters: at most
 // 8 * 2^28 = 2^31 increments between folds.
 constexpr uint32_t FLUSH_EVERY = 1u << 28;
+// Trips between folds of the wave-uniform 32-bit counter (64 lanes add to it).
+constexpr uint32_t FLUSH_EVERY_WAVE = 1u << 22;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
@@ -101,14 +105,22 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 //
 //   p = v & -v; v ^= p                          lowest candidate, v = candidates left
 //   push {c,l,r,v} if v != 0                    (masked, or unconditional: see below)
-//   c |= p; l = (l|p) << 1; r = (r|p) >> 1
+//   c |= p; l = (l|p) << 1; r = (r|p) >> 1      (l & p == 0: one v_add_lshl_u32)
 //   v = ~(l | r | c)                            (v_bitop3 table 0x01)
-//   if (c == ~0) cnt += 1                       (compare + add with carry, no mask switch)
-//   if (v == 0) { if (sp != base) { sp -= stride; {c,l,r,v} = stack[sp] } else done |= lane }
+//   if (c == ~0) cnt += 1                       (compare + add with carry, or a scalar
+//                                                popcount of the compare mask per wave)
+//   if (v == 0) { sp = max(sp - stride, 0); {c,l,r,v} = lds[sp] }
+//
+// The pop needs no stack-empty test.  LDS address 0 holds a dead frame DEAD_FRAME and every
+// lane's level 0 lies less than one stride above it, so the saturating subtract (v_sub_u32
+// with clamp) sends the pop of an empty stack to address 0.  A lane on the dead frame places
+// nothing (p = 0), its child is dead again (~(c | l<<1 | r>>1) == 0, c != ~0: nothing is
+// counted) and it pops the dead frame again, until the end of the trip (eight iterations),
+// where lanes with sp == 0 are found and the asm statement returns for their refill.  Done
+// lanes thus idle until the end of the trip, as they would out of exec.
 //
 // Only SALU instructions read the masks the VALU compares write, and every LDS read is waited
-// for before the iteration ends, so no hazard wait states are needed.  Lanes that become done
-// are removed from exec for the rest of the asm statement.
+// for before the iteration ends, so no hazard wait states are needed.
 #define NQ_EXTRACT                                                                            \
     "v_sub_u32 %[t], 0, v23\n\t"                                                              \
     "v_and_b32 %[p], v23, %[t]\n\t"                                                           \
@@ -124,46 +136,51 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 
 // Write every lane's frame and keep it (advance sp by stride * min(v, 1)) only where
 // candidates remain: no exec switch.  A lane whose every level is in use writes one level
-// above its top frame, so this form needs one spare level.
+// above its top frame, so this form needs one spare level.  A lane on the dead frame writes
+// the dead frame back to address 0 unchanged (v == 0 after a placement of p = 0).
 #define NQ_PUSH_ALWAYS                                                                        \
     "ds_write_b128 %[sp], v[20:23]\n\t"                                                       \
     "v_min_u32 %[t], 1, v23\n\t"                                                              \
     "v_mad_u32_u24 %[sp], %[t], %[stride], %[sp]\n\t"
 
-#define NQ_CHILD_POP                                                                          \
+// LANE_CNT adds the completed board (vcc) to the lane's counter; WAVE_CNT adds the popcount of
+// vcc to a wave-uniform SGPR counter and sits in the shadow of the pop's LDS read.
+#define NQ_CHILD_POP(LANE_CNT, WAVE_CNT)                                                      \
     "v_or_b32 v20, v20, %[p]\n\t"                                                             \
-    "v_or_b32 v21, v21, %[p]\n\t"                                                             \
-    "v_lshlrev_b32 v21, 1, v21\n\t"                                                           \
+    "v_add_lshl_u32 v21, v21, %[p], 1\n\t"                                                    \
     "v_or_b32 v22, v22, %[p]\n\t"                                                             \
     "v_lshrrev_b32 v22, 1, v22\n\t"                                                           \
     "v_bitop3_b32 v23, v21, v22, v20 bitop3:1\n\t"                                            \
-    "v_cmp_eq_u32 vcc, -1, v20\n\t"                                                           \
-    "v_addc_co_u32 %[cnt], vcc, 0, %[cnt], vcc\n\t"                                           \
+    "v_cmp_eq_u32 vcc, -1, v20\n\t" LANE_CNT                                                  \
     "v_cmp_eq_u32 %[m1], 0, v23\n\t"                                                          \
-    "v_cmp_eq_u32 %[m2], %[sp], %[base]\n\t"                                                  \
-    "s_and_b64 %[m2], %[m1], %[m2]\n\t"                                                       \
-    "s_andn2_b64 %[m1], %[m1], %[m2]\n\t"                                                     \
-    "s_or_b64 %[done], %[done], %[m2]\n\t"                                                    \
     "s_and_saveexec_b64 %[tmp], %[m1]\n\t"                                                    \
-    "v_subrev_u32 %[sp], %[stride], %[sp]\n\t"                                                \
+    "v_sub_u32_e64 %[sp], %[sp], %[stride] clamp\n\t"                                         \
     "ds_read_b128 v[20:23], %[sp]\n\t"                                                        \
-    "s_andn2_b64 exec, %[tmp], %[m2]\n\t"                                                     \
+    "s_mov_b64 exec, %[tmp]\n\t" WAVE_CNT                                                     \
     "s_waitcnt lgkmcnt(0)\n\t"
 
-#define NQ_ITER(PUSH, NODES_INSN) NQ_EXTRACT PUSH NODES_INSN NQ_CHILD_POP
-#define NQ_ITER2(PUSH, NODES_INSN) NQ_ITER(PUSH, NODES_INSN) NQ_ITER(PUSH, NODES_INSN)
+#define NQ_CNT_LANE "v_addc_co_u32 %[cnt], vcc, 0, %[cnt], vcc\n\t"
+#define NQ_CNT_WAVE                                                                           \
+    "s_bcnt1_i32_b64 %[bc], vcc\n\t"                                                          \
+    "s_add_u32 %[wcnt], %[wcnt], %[bc]\n\t"
 
-// The whole hot loop: eight iterations per trip, leaving when some lane becomes done or after
-// [iters] trips (the caller folds the 32-bit counters then).  [nd] counts expansions (node-
-// counting variant only).
-#define NQ_HOT_ASM(PUSH, NODES_INSN)                                                          \
+#define NQ_ITER(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
+    NODES_INSN NQ_EXTRACT PUSH NQ_CHILD_POP(LANE_CNT, WAVE_CNT)
+#define NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
+    NQ_ITER(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)
+
+// The whole hot loop: eight iterations per trip, leaving after a trip that ends with some lane
+// on the dead frame (sp == 0), or after [iters] trips (the caller folds the 32-bit counters
+// then).  [nd] counts expansions (node-counting variant only).
+#define NQ_HOT_ASM(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)                                      \
     __asm__ volatile(                                                                         \
         "s_mov_b64 %[save], exec\n\t"                                                         \
-        "s_mov_b64 exec, %[live]\n\t"                                                         \
-        "s_mov_b64 %[done], 0\n"                                                              \
-        "nq_hot_%=:\n\t" NQ_ITER2(PUSH, NODES_INSN) NQ_ITER2(PUSH, NODES_INSN)                \
-        NQ_ITER2(PUSH, NODES_INSN) NQ_ITER2(PUSH, NODES_INSN)                                 \
-        "s_cmp_lg_u64 %[done], 0\n\t"                                                         \
+        "s_mov_b64 exec, %[live]\n"                                                           \
+        "nq_hot_%=:\n\t" NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)                        \
+        NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
+        NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)                                         \
+        "v_cmp_eq_u32 %[m1], 0, %[sp]\n\t"                                                    \
+        "s_cmp_lg_u64 %[m1], 0\n\t"                                                           \
         "s_cbranch_scc1 nq_exit_%=\n\t"                                                       \
         "s_sub_u32 %[iters], %[iters], 1\n\t"                                                 \
         "s_cmp_lg_u32 %[iters], 0\n\t"                                                        \
@@ -171,32 +188,75 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
         "nq_exit_%=:\n\t"                                                                     \
         "s_mov_b64 exec, %[save]\n\t"                                                         \
         : "+{v20}"(c), "+{v21}"(l), "+{v22}"(r), "+{v23}"(v), [sp] "+v"(sp), [cnt] "+v"(cnt),   \
-          [nd] "+v"(nd), [iters] "+s"(iters), [t] "=&v"(t), [p] "=&v"(p), [m1] "=&s"(m1),       \
-          [m2] "=&s"(m2), [tmp] "=&s"(tmp), [save] "=&s"(save), [done] "=&s"(done)             \
-        : [live] "s"(live), [base] "v"(base), [stride] "s"(stride)                            \
+          [nd] "+v"(nd), [iters] "+s"(iters), [wcnt] "+s"(wcnt), [t] "=&v"(t), [p] "=&v"(p),   \
+          [m1] "=&s"(m1), [tmp] "=&s"(tmp), [save] "=&s"(save), [bc] "=&s"(bc)                  \
+        : [live] "s"(live), [stride] "s"(stride)                                              \
         : "memory", "scc", "vcc")
 
-#define NQ_NODE_INC "v_add_u32 %[nd], 1, %[nd]\n\t"
+// Expansions of the lanes not on the dead frame (sp != 0 at the start of the iteration).
+#define NQ_NODE_INC                                                                           \
+    "v_min_u32 %[t], 1, %[sp]\n\t"                                                            \
+    "v_add_u32 %[nd], %[t], %[nd]\n\t"
 
 // Runs the hot loop for the lanes in `live` until one of them finishes its task (or the fold
-// interval ends).  APUSH selects the unconditional push (the stack has a spare level).
-template <bool NODES, bool APUSH>
+// interval ends).  APUSH selects the unconditional push (the stack has a spare level).  WAVE
+// counts completed boards per wave into wcnt (SALU) instead of per lane into cnt; its fold
+// interval is 2^22 trips, at most 2^22 * 8 * 64 = 2^31 boards.
+template <bool NODES, bool APUSH, bool WAVE>
 __device__ __forceinline__ void hot_loop(uint32_t &c, uint32_t &l, uint32_t &r, uint32_t &v, uint32_t &sp,
-                                         uint32_t &cnt, uint32_t &nd, uint64_t live, uint32_t base,
+                                         uint32_t &cnt, uint32_t &wcnt, uint32_t &nd, uint64_t live,
                                          uint32_t stride) {
-    uint32_t t, p, iters = FLUSH_EVERY;
-    uint64_t m1, m2, tmp, save, done;
-    if (APUSH) {
-        if (NODES)
-            NQ_HOT_ASM(NQ_PUSH_ALWAYS, NQ_NODE_INC);
+    uint32_t t, p, bc, iters = WAVE ? FLUSH_EVERY_WAVE : FLUSH_EVERY;
+    uint64_t m1, tmp, save;
+    if (WAVE) {
+        static_assert(!WAVE || !NODES, "the node-counting variant counts per lane");
+        if (APUSH)
+            NQ_HOT_ASM(NQ_PUSH_ALWAYS, "", "", NQ_CNT_WAVE);
         else
-            NQ_HOT_ASM(NQ_PUSH_ALWAYS, "");
+            NQ_HOT_ASM(NQ_PUSH_MASKED, "", "", NQ_CNT_WAVE);
+    } else if (APUSH) {
+        if (NODES)
+            NQ_HOT_ASM(NQ_PUSH_ALWAYS, NQ_NODE_INC, NQ_CNT_LANE, "");
+        else
+            NQ_HOT_ASM(NQ_PUSH_ALWAYS, "", NQ_CNT_LANE, "");
     } else {
         if (NODES)
-            NQ_HOT_ASM(NQ_PUSH_MASKED, NQ_NODE_INC);
+            NQ_HOT_ASM(NQ_PUSH_MASKED, NQ_NODE_INC, NQ_CNT_LANE, "");
         else
-            NQ_HOT_ASM(NQ_PUSH_MASKED, "");
+            NQ_HOT_ASM(NQ_PUSH_MASKED, "", NQ_CNT_LANE, "");
     }
+}
+
+// Dynamic LDS of dfs_kernel and batch_kernel (the kernels have no static LDS, so it starts at
+// address 0): the dead frame, the frame stack [levels][blockDim.x], then per wave the claim
+// range wq[2] and the reduction slot red[2].
+struct LdsLayout {
+    uint4 *stack;
+    unsigned long long (*wq)[2];
+    unsigned long long (*red)[2];
+};
+
+__device__ __forceinline__ LdsLayout lds_layout(uint4 *lds, uint32_t levels) {
+    LdsLayout L;
+    L.stack = lds + 1;
+    auto *tail = reinterpret_cast<unsigned long long(*)[2]>(lds + 1 + levels * blockDim.x);
+    L.wq = tail;
+    L.red = tail + MAX_WAVES_PER_BLOCK;
+    return L;
+}
+
+// {c, l, r, v} of the dead frame: p = 0 leaves it as is, its child ~(c | l<<1 | r>>1) is 0 and
+// c != ~0.  Written by lane 0 of every wave before the wave's first pop (identical values).
+__device__ __forceinline__ void put_dead_frame(uint4 *lds) {
+    if ((threadIdx.x & 63u) == 0) lds[0] = make_uint4(0xFFFFFFFEu, 0u, 0xFFFFFFFFu, 0u);
+}
+
+// The dead frame must sit at LDS address 0 (the target of the saturating pop).  Nonzero only if
+// the kernel were given static LDS; then it flags DEV_ERR_LDS and does nothing.
+__device__ __forceinline__ bool lds_base_ok(const uint4 *lds, DevState *st) {
+    if (static_cast<uint32_t>(reinterpret_cast<uintptr_t>(lds)) == 0u) return true;
+    if (threadIdx.x == 0) atomicOr(&st->error, DEV_ERR_LDS);
+    return false;
 }
 
 template <bool NODES, bool PER_TASK, bool APUSH>
@@ -206,9 +266,11 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
            uint64_t *__restrict__ trace, const unsigned long long *__restrict__ ntasks_dev) {
     // Task count produced on the device (expansion), never beyond the buffer bound `ntasks`.
     if (ntasks_dev) ntasks = min(ntasks, static_cast<uint64_t>(*ntasks_dev));
-    extern __shared__ uint4 stack[];  // [stack_depth + APUSH][blockDim.x]
-    __shared__ unsigned long long wq[MAX_WAVES_PER_BLOCK][2];
-    __shared__ unsigned long long red[MAX_WAVES_PER_BLOCK][2];
+    extern __shared__ uint4 lds[];  // lds_layout(): dead frame, stack, wq, red
+    if (!lds_base_ok(lds, st)) return;
+    const LdsLayout L = lds_layout(lds, stack_depth + (APUSH ? 1u : 0u));
+    auto *wq = L.wq;
+    auto *red = L.red;
 
     const uint32_t wave = threadIdx.x >> 6;
     const uint64_t t_start = trace ? wall_clock64() : 0;
@@ -216,13 +278,20 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
         wq[wave][0] = 0;
         wq[wave][1] = 0;
     }
+    put_dead_frame(lds);
     const uint32_t last = (1u << N) - 1u;
     const uint32_t stride = blockDim.x * static_cast<uint32_t>(sizeof(uint4));
-    const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(stack + threadIdx.x));
+    // Level 0 of this lane; a pop from it saturates to the dead frame at address 0.
+    const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(L.stack + threadIdx.x));
     uint32_t sp = base;
 
     uint32_t c = 0, l = 0, r = 0, v = 0;  // the frame being expanded
+    // Per-task counts and node counts need a counter per lane; a plain total is counted per
+    // wave (SALU popcount of the completed-board mask, off the VALU).
+    constexpr bool WAVE = !NODES && !PER_TASK;
     uint32_t cnt = 0;     // completed boards since the last fold (at most one per iteration)
+    uint32_t wcnt = 0;    // the same for the whole wave (WAVE variant), wave-uniform
+    uint64_t wsum = 0;    // folded wcnt, wave-uniform
     uint32_t nd = 0;      // expansions since the last fold (node-counting variant)
     uint64_t tsum = 0;    // folded count of the current task
     uint64_t acc = 0;     // counts of finished tasks
@@ -257,7 +326,7 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
                 }
                 if (N - 1u - d0 > stack_depth) {
                     // Not enough frames for this prefix: flag and skip (host checks first).
-                    atomicOr(&st->error, 1ull);
+                    atomicOr(&st->error, DEV_ERR_DEPTH);
                     if (PER_TASK) per_task[t] = 0;
                     continue;
                 }
@@ -265,6 +334,7 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
                 l = l0;
                 r = r0;
                 v = v0;
+                sp = base;
                 tid = t;
                 has = true;
                 break;
@@ -273,10 +343,12 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
         const uint64_t live = __ballot(has);
         if (live == 0) break;
         // Hot loop until a lane needs a new task.
-        hot_loop<NODES, APUSH>(c, l, r, v, sp, cnt, nd, live, base, stride);
+        hot_loop<NODES, APUSH, WAVE>(c, l, r, v, sp, cnt, wcnt, nd, live, stride);
         tsum += cnt;
         if (NODES) nodes += static_cast<uint64_t>(nd) - cnt;
+        wsum += wcnt;
         cnt = 0;
+        wcnt = 0;
         nd = 0;
     }
 
@@ -287,7 +359,7 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
         trace[2 * w + 1] = wall_clock64();
     }
     // Reduce: wave (shuffles), workgroup (LDS), device (one atomic per workgroup).
-    acc = wave_sum_u64(acc);
+    acc = wave_sum_u64(acc) + wsum;
     if (NODES) nodes = wave_sum_u64(nodes);
     if ((threadIdx.x & 63u) == 0) {
         red[wave][0] = acc;
@@ -316,17 +388,20 @@ template <bool APUSH>
 __global__ void __launch_bounds__(MAX_BLOCK)
 batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, uint32_t stack_depth,
              DevState *__restrict__ st, unsigned long long *__restrict__ counts) {
-    extern __shared__ uint4 stack[];
-    __shared__ unsigned long long wq[MAX_WAVES_PER_BLOCK][2];
-    __shared__ unsigned long long red[MAX_WAVES_PER_BLOCK];
+    extern __shared__ uint4 lds[];  // lds_layout(): dead frame, stack, wq, red
+    if (!lds_base_ok(lds, st)) return;
+    const LdsLayout L = lds_layout(lds, stack_depth + (APUSH ? 1u : 0u));
+    auto *wq = L.wq;
+    auto *red = L.red;
 
     const uint32_t wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 0) {
         wq[wave][0] = 0;
         wq[wave][1] = 0;
     }
+    put_dead_frame(lds);
     const uint32_t stride = blockDim.x * static_cast<uint32_t>(sizeof(uint4));
-    const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(stack + threadIdx.x));
+    const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(L.stack + threadIdx.x));
     uint32_t sp = base;
     uint32_t c = 0, l = 0, r = 0, v = 0, cnt = 0, nd = 0;
     uint32_t inst = 0, weight = 1;
@@ -367,13 +442,14 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
                     continue;
                 }
                 if (N - 1u - d0 > stack_depth) {
-                    atomicOr(&st->error, 1ull);
+                    atomicOr(&st->error, DEV_ERR_DEPTH);
                     continue;
                 }
                 c = w.x | ~last;
                 l = w.y;
                 r = w.z;
                 v = v0;
+                sp = base;
                 has = true;
                 own = true;
                 break;
@@ -381,17 +457,18 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
         }
         const uint64_t live = __ballot(has);
         if (live == 0) break;
-        hot_loop<false, APUSH>(c, l, r, v, sp, cnt, nd, live, base, stride);
+        uint32_t wcnt = 0;  // unused: a batch counts per lane (lanes work on different instances)
+        hot_loop<false, APUSH, false>(c, l, r, v, sp, cnt, wcnt, nd, live, stride);
         tsum += cnt;
         cnt = 0;
     }
     if (run_sum) atomicAdd(&counts[run_inst], static_cast<unsigned long long>(run_sum));
     acc = wave_sum_u64(acc);
-    if ((threadIdx.x & 63u) == 0) red[wave] = acc;
+    if ((threadIdx.x & 63u) == 0) red[wave][0] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long s = 0;
-        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) s += red[w];
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) s += red[w][0];
         atomicAdd(&st->total, s);
     }
 }
@@ -585,7 +662,9 @@ hipError_t launch_batch(const BatchArgs &a, hipStream_t stream) {
 }
 
 size_t dfs_lds_bytes(int block, int levels) {
-    return static_cast<size_t>(block) * static_cast<size_t>(levels) * sizeof(uint4);
+    // Dead frame + stack + per-wave claim range and reduction slot (lds_layout()).
+    return sizeof(uint4) * (1 + static_cast<size_t>(block) * static_cast<size_t>(levels)) +
+           2 * MAX_WAVES_PER_BLOCK * 2 * sizeof(unsigned long long);
 }
 
 hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task, bool apush,

@@ -17,8 +17,12 @@ struct DevState {
     unsigned long long cursor;  // next unclaimed task index
     unsigned long long total;   // sum of per-task completion counts
     unsigned long long nodes;   // DFS nodes (node-counting kernel only)
-    unsigned long long error;   // nonzero: a task needed more stack frames than configured
+    unsigned long long error;   // DEV_ERR_* bits
 };
+
+// DevState::error bits (folded into result[2] beside the expansion's overflow bit 2).
+constexpr unsigned long long DEV_ERR_DEPTH = 1;  // a task needed more stack frames than configured
+constexpr unsigned long long DEV_ERR_LDS = 4;    // dynamic LDS did not start at address 0
 
 struct LaunchArgs {
     const uint32_t *tasks;  // 3 words per task: cur, left, right

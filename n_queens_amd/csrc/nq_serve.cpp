@@ -74,6 +74,7 @@ struct Slot {
     uint64_t *d_counts = nullptr;
     uint64_t *h_counts = nullptr;  // pinned
     DevState *d_state = nullptr;
+    DevState *h_state = nullptr;  // pinned copy of the batch's flags
     std::vector<Slice> slices;
     double t_launch = 0;
     bool busy = false;
@@ -106,6 +107,7 @@ static void free_slot(Slot &s) {
     if (s.d_state) (void)hipFree(s.d_state);
     if (s.h_tasks) (void)hipHostFree(s.h_tasks);
     if (s.h_counts) (void)hipHostFree(s.h_counts);
+    if (s.h_state) (void)hipHostFree(s.h_state);
     if (s.done_ev) (void)hipEventDestroy(s.done_ev);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     s = Slot();
@@ -185,6 +187,7 @@ static hipError_t launch_slot(nq_server *S, Slot &s) {
     if (e == hipSuccess)
         e = hipMemcpyAsync(s.h_counts, s.d_counts, s.slices.size() * sizeof(uint64_t), hipMemcpyDeviceToHost,
                            s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.h_state, s.d_state, sizeof(DevState), hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipEventRecord(s.done_ev, s.stream);
     return e;
 }
@@ -211,6 +214,11 @@ static void scheduler(nq_server *S) {
             if (q != hipSuccess) {
                 S->error = NQ_ERR_HIP;
                 S->error_msg = "serving batch failed";
+            } else if (s.h_state->error) {
+                // Admission sizes every request's frames, so this is a kernel fault, not a user error.
+                S->error = NQ_ERR_STATE;
+                S->error_msg = (s.h_state->error & DEV_ERR_LDS) ? "batch kernel LDS does not start at address 0"
+                                                                : "a served task needed more stack frames than sized";
             }
             for (size_t i = 0; i < s.slices.size(); ++i) {
                 Request &r = *s.slices[i].req;
@@ -313,6 +321,7 @@ int nq_server_open(const nq_server_config *cfg_in, nq_server **out) {
         if (e == hipSuccess) e = hipMalloc(&s.d_counts, cfg.slot_requests * sizeof(uint64_t));
         if (e == hipSuccess) e = hipHostMalloc(&s.h_counts, cfg.slot_requests * sizeof(uint64_t), hipHostMallocDefault);
         if (e == hipSuccess) e = hipMalloc(&s.d_state, sizeof(DevState));
+        if (e == hipSuccess) e = hipHostMalloc(&s.h_state, sizeof(DevState), hipHostMallocDefault);
     }
     if (e != hipSuccess) {
         for (Slot &s : S->slots) free_slot(s);
