@@ -138,13 +138,13 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
         return fail(NQ_ERR_DEPTH, buf);
     }
     // Resident workgroups per CU for both push forms (capped by the waves_per_cu target); the
-    // unconditional push is taken when its spare stack level costs no workgroup.
+    // unconditional push is taken unless its spare stack level costs over 15% of them.
     int occ[2] = {0, 0};
     NQ_HIP(dfs_occupancy(c.block, c.stack_depth, nodes, per_task, false, &occ[0]));
     NQ_HIP(dfs_occupancy(c.block, c.stack_depth, nodes, per_task, true, &occ[1]));
     if (c.waves_per_cu > 0)
         for (int &o : occ) o = std::min(o, std::max(1, c.waves_per_cu / (c.block / 64)));
-    if (c.push == 0) c.push = (occ[1] > 0 && occ[1] >= occ[0]) ? NQ_PUSH_ALWAYS : NQ_PUSH_MASKED;
+    if (c.push == 0) c.push = prefer_always_push(occ[0], occ[1]) ? NQ_PUSH_ALWAYS : NQ_PUSH_MASKED;
     const int bpc = occ[c.push == NQ_PUSH_ALWAYS ? 1 : 0];
     if (bpc <= 0) return fail(NQ_ERR_ARG, "configuration does not fit on a CU (LDS)");
     if (c.grid == 0) c.grid = d->cus * bpc;

@@ -21,7 +21,8 @@
 //    so lanes of one wave may work on different board sizes (batch_kernel).
 //  * Two push forms: exec-masked (only frames with candidates left are written), or an
 //    unconditional write with sp advanced by stride * min(v, 1) (no exec switch, one spare
-//    stack level); the host takes the second when the spare level costs no occupancy.
+//    stack level); the host takes the second unless the spare level costs more than 15% of
+//    the resident waves (prefer_always_push).
 //  * A lane whose task is exhausted claims the next one at once, in the same loop (lane-
 //    granular refill), from a global cursor; claims are aggregated per wave and handed out in
 //    chunks through a wave-private LDS range.
@@ -637,7 +638,7 @@ hipError_t batch_fit(int block, int stack_depth, int waves_per_cu, int push, boo
     if (e != hipSuccess) return e;
     if (waves_per_cu > 0)
         for (int &o : occ) o = std::min(o, std::max(1, waves_per_cu / (block / 64)));
-    *apush = push == 2 || (push == 0 && occ[1] > 0 && occ[1] >= occ[0]);
+    *apush = push == 2 || (push == 0 && prefer_always_push(occ[0], occ[1]));
     *blocks_per_cu = occ[*apush ? 1 : 0];
     return hipSuccess;
 }
