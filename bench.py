@@ -282,14 +282,22 @@ def main() -> int:
     nodes_per_s = nodes / (ms_per_step / 1e3)
     peak = info["cus"] * LANES_PER_CU * PEAK_CLOCK_HZ / 1e12
     achieved = nodes * OPS_PER_NODE / (ms_per_step / 1e3) / 1e12
+    # The DFS launch of a timed step as the library resolves it (the split tasks start at depth
+    # rows + split): workgroup size, push form, grid.
+    split = max(0, cfg.split)
+    launch = nq.resolve_config(N, min(N, rows + split), 1 << 40, nq.Config(
+        block=cfg.block, stack_depth=max(1, cfg.stack_depth - split), chunk=cfg.chunk, push=cfg.push,
+        waves_per_cu=cfg.waves_per_cu))
 
     out = {
         "metric": METRIC, "value": nodes_per_s, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "strong",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"N={N} whole-board count, {rows} rows pre-placed (configuration2)", "N": N,
-                   "rows": rows, "ntasks": ntasks, "block": cfg.block, "stack_depth": cfg.stack_depth,
-                   "chunk": cfg.chunk, "device_split_rows": cfg.split,
+                   "rows": rows, "ntasks": ntasks, "block": launch.block, "dfs_stack_frames": launch.stack_depth,
+                   "push": "unconditional" if launch.push == nq.NQ_PUSH_ALWAYS else "masked",
+                   "waves_per_cu": launch.grid * (launch.block // 64) // info["cus"],
+                   "chunk": launch.chunk, "device_split_rows": cfg.split,
                    "parallelism": "single GPU, persistent grid"},
         "solutions": 2 * sum0, "nodes": nodes,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tops/s (int32 lane-ops)",

@@ -36,7 +36,9 @@ enum nq_status {
  * CONFIG1 128 threads / depth 24, CONFIG2 160 / 19, CONFIG3 192 / 16) plus grid 1024
  * (src/n_queens_cuda.cu:164-167). */
 typedef struct nq_config {
-    int block;        /* threads per workgroup, a multiple of 64, <= 256                  */
+    int block;        /* threads per workgroup, a multiple of 64, <= 256; 0 = automatic:  */
+                      /* 256 when that keeps the resident waves per CU that 128 gets with   */
+                      /* the same push rule (one wave of each workgroup per SIMD), else 128 */
     int stack_depth;  /* LDS frames per lane; must be >= N - 1 - (prefix depth)           */
     int waves_per_cu; /* resident-wave target that sizes the persistent grid (0 = max)     */
     int chunk;        /* tasks a wave claims per cursor grab (1..4096)                    */
@@ -74,16 +76,25 @@ int nq_set_device(int device);        /* cudaSetDevice, n_queens_cuda.cu:149    
 /* Compute units, reported clock (kHz) and "name (gcnArch)" of a device. */
 int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t name_len);
 
-/* Fill cfg with defaults for (N, shallowest prefix depth d0): 128-thread workgroups, the
- * smallest legal stack depth N-1-d0 (LDS per lane decides occupancy, so no frame is wasted),
- * chunk 16, push NQ_PUSH_AUTO, and a persistent grid of every workgroup that fits on the device.  The reference's
+/* Fill cfg with defaults for (N, shallowest prefix depth d0): automatic workgroup size (block
+ * 0), the smallest legal stack depth N-1-d0 (LDS per lane decides occupancy, so no frame is
+ * wasted), chunk 16, push NQ_PUSH_AUTO, and a persistent grid of every workgroup that fits on
+ * the device.  The reference's
  * compile-time presets only decide which (N, rows) pairs its CLI accepts; the CLI here keeps
  * that check (src/main.cpp:16-19). */
 int nq_config_default(int N, int d0, nq_config *cfg);
 
 /* The configuration nq_solve(N, rows, NULL, ...) uses: defaults for the tasks after the
- * automatic device split (one more row when 14 or more rows remain below the prefix). */
+ * automatic device split of clamp(N - rows - (N+4)/2, 0, 3) rows (the DFS starts (N+4)/2 rows
+ * above the last row). */
 int nq_solve_config(int N, int rows, nq_config *cfg);
+
+/* The launch configuration nq_count takes on the current device for ntasks tasks whose
+ * shallowest prefix depth is d0 (after any device split; plain totals, no per-task or node
+ * counts): cfg's zero fields resolved, i.e. the workgroup size, the push form (never
+ * NQ_PUSH_AUTO in *out) and the persistent grid.  No reference counterpart: it reports what the
+ * reference fixes at compile time (src/macro.h:25-35, src/n_queens_cuda.cu:164-167). */
+int nq_config_resolve(int N, int d0, uint64_t ntasks, const nq_config *cfg, nq_config *out);
 
 /* ---- prefix/task generator (host) ------------------------------------------------------
  * Replaces partial_n_queens + partial_n_queens_for_odd (src/n_queens.cpp:74-113) as called by

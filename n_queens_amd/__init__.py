@@ -35,7 +35,7 @@ NQ_PUSH_AUTO, NQ_PUSH_MASKED, NQ_PUSH_ALWAYS = 0, 1, 2
 # Every symbol include/nq.h declares (checked by tests/test_abi.py).
 EXPORTS = (
     "nq_device_count", "nq_last_error", "nq_set_device", "nq_device_info", "nq_config_default",
-    "nq_generate", "nq_count", "nq_count_device", "nq_solve", "nq_solve_config",
+    "nq_generate", "nq_count", "nq_count_device", "nq_solve", "nq_solve_config", "nq_config_resolve",
     "nq_batch_plan_create", "nq_batch_plan_launch", "nq_batch_plan_wait", "nq_batch_plan_run",
     "nq_batch_plan_destroy", "nq_batch_count", "nq_instance_count",
     "nq_server_config_default", "nq_server_open", "nq_server_submit", "nq_server_poll", "nq_server_wait",
@@ -122,6 +122,8 @@ def lib() -> ctypes.CDLL:
                                  ctypes.c_char_p, ctypes.c_size_t]
     L.nq_config_default.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Config)]
     L.nq_solve_config.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Config)]
+    L.nq_config_resolve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_Config),
+                                    ctypes.POINTER(_Config)]
     L.nq_generate.argtypes = [ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_uint64, _u64p]
     L.nq_count.argtypes = [ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.POINTER(_Config), _u64p, _u64p, _u64p,
                            ctypes.POINTER(_Stats)]
@@ -180,6 +182,15 @@ def solve_config(N: int, rows: int) -> Config:
     """The configuration nq_solve uses for (N, rows), including its device split."""
     c = _Config()
     _check(lib().nq_solve_config(N, rows, ctypes.byref(c)))
+    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push)
+
+
+def resolve_config(N: int, d0: int, ntasks: int, cfg: Config | None = None) -> Config:
+    """The launch configuration nq_count takes on this device for `ntasks` tasks of shallowest
+    prefix depth d0 (after any split): workgroup size, push form and grid resolved."""
+    c = _Config()
+    cc = cfg._c() if cfg else None
+    _check(lib().nq_config_resolve(N, d0, ntasks, ctypes.byref(cc) if cc else None, ctypes.byref(c)))
     return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push)
 
 

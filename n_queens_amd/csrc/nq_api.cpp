@@ -104,7 +104,8 @@ static int current_device(Device **out) {
 }
 
 static int check_cfg_shape(const nq_config &c) {
-    if (c.block < 64 || c.block > MAX_BLOCK || c.block % 64) return fail(NQ_ERR_ARG, "block must be 64..256, multiple of 64");
+    if (c.block != 0 && (c.block < 64 || c.block > MAX_BLOCK || c.block % 64))
+        return fail(NQ_ERR_ARG, "block must be 0 (automatic) or 64..256, multiple of 64");
     if (c.chunk < 1 || c.chunk > 4096) return fail(NQ_ERR_ARG, "chunk must be 1..4096");
     if (c.stack_depth < 1 || c.stack_depth > 32) return fail(NQ_ERR_ARG, "stack_depth must be 1..32");
     if (c.waves_per_cu < 0 || c.grid < 0) return fail(NQ_ERR_ARG, "negative waves_per_cu/grid");
@@ -137,15 +138,39 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
                  N, d0, frames_needed(N, d0));
         return fail(NQ_ERR_DEPTH, buf);
     }
-    // Resident workgroups per CU for both push forms (capped by the waves_per_cu target); the
-    // unconditional push is taken unless its spare stack level costs over 15% of them.
-    int occ[2] = {0, 0};
-    NQ_HIP(dfs_occupancy(c.block, c.stack_depth, nodes, per_task, false, &occ[0]));
-    NQ_HIP(dfs_occupancy(c.block, c.stack_depth, nodes, per_task, true, &occ[1]));
-    if (c.waves_per_cu > 0)
-        for (int &o : occ) o = std::min(o, std::max(1, c.waves_per_cu / (c.block / 64)));
-    if (c.push == 0) c.push = prefer_always_push(occ[0], occ[1]) ? NQ_PUSH_ALWAYS : NQ_PUSH_MASKED;
-    const int bpc = occ[c.push == NQ_PUSH_ALWAYS ? 1 : 0];
+    // Resident workgroups per CU of a workgroup size for both push forms (capped by the
+    // waves_per_cu target); the unconditional push is taken unless its spare stack level costs
+    // over 15% of them.
+    auto fit = [&](int block, int *push, int *bpc) -> int {
+        int occ[2] = {0, 0};
+        NQ_HIP(dfs_occupancy(block, c.stack_depth, nodes, per_task, false, &occ[0]));
+        NQ_HIP(dfs_occupancy(block, c.stack_depth, nodes, per_task, true, &occ[1]));
+        if (c.waves_per_cu > 0)
+            for (int &o : occ) o = std::min(o, std::max(1, c.waves_per_cu / (block / 64)));
+        *push = c.push ? c.push : (prefer_always_push(occ[0], occ[1]) ? NQ_PUSH_ALWAYS : NQ_PUSH_MASKED);
+        *bpc = occ[*push == NQ_PUSH_ALWAYS ? 1 : 0];
+        return NQ_OK;
+    };
+    // Automatic workgroup size: 256 lanes (one wave of each workgroup per SIMD) when that keeps
+    // the resident waves 128 lanes get, else 128.  With equal waves 256 is about 0.5% faster
+    // (N=20 and N=21, 6 rows, 12 waves/CU); where 128 fits an odd number of workgroups, 256
+    // would lose waves (N=20 after a 3-row split: 14 against 12).
+    const bool auto_block = c.block == 0;
+    if (auto_block) c.block = 128;
+    int push = 0, bpc = 0;
+    rc = fit(c.block, &push, &bpc);
+    if (rc) return rc;
+    if (auto_block) {
+        int push256 = 0, bpc256 = 0;
+        rc = fit(256, &push256, &bpc256);
+        if (rc) return rc;
+        if (bpc256 > 0 && bpc256 * 4 >= bpc * 2) {
+            c.block = 256;
+            push = push256;
+            bpc = bpc256;
+        }
+    }
+    c.push = push;
     if (bpc <= 0) return fail(NQ_ERR_ARG, "configuration does not fit on a CU (LDS)");
     if (c.grid == 0) c.grid = d->cus * bpc;
     const uint64_t lanes_needed = (ntasks + c.block - 1) / c.block;
@@ -427,7 +452,7 @@ int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t 
 
 int nq_config_default(int N, int d0, nq_config *cfg) {
     if (!cfg) return fail(NQ_ERR_ARG, "cfg is NULL");
-    cfg->block = 128;
+    cfg->block = 0;  // automatic (resolve_config)
     cfg->stack_depth = std::max(1, frames_needed(N, d0));
     cfg->waves_per_cu = 0;
     cfg->chunk = 16;
@@ -583,6 +608,16 @@ int nq_solve_config(int N, int rows, nq_config *cfg) {
     nq_config_default(N, std::min(N, rows + split), cfg);
     cfg->split = split;
     return NQ_OK;
+}
+
+int nq_config_resolve(int N, int d0, uint64_t ntasks, const nq_config *cfg, nq_config *out) {
+    if (!out) return fail(NQ_ERR_ARG, "out is NULL");
+    if (N < 1 || N > MAX_N || d0 < 0 || d0 > N) return fail(NQ_ERR_ARG, "N must be 1..28 and d0 0..N");
+    Device *d = nullptr;
+    int rc = current_device(&d);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(d->mu);
+    return resolve_config(d, N, d0, ntasks, cfg, false, false, out);
 }
 
 int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats) {

@@ -220,6 +220,7 @@ int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, n
     const double t0 = now_ms();
     nq_config c;
     nq_config_default(MAX_N, 0, &c);
+    c.block = 128;  // batches keep 128-lane workgroups
     c.stack_depth = 0;
     if (cfg) {
         if (cfg->block) c.block = cfg->block;

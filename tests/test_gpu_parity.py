@@ -188,6 +188,22 @@ def test_batch_chunked_claims(chunk):
             assert got == oracle.count_subtree(N, *oracle.encode_prefix(N, cols)), (N, cols)
 
 
+@pytest.mark.gpu
+def test_automatic_block_and_resolved_launch(published):
+    """Block 0 resolves to 256 when that keeps the resident waves of 128, else 128; explicit
+    blocks are kept; counts do not depend on it."""
+    big = 1 << 40
+    r21 = nq.resolve_config(21, 9, big)  # N=21 after a 3-row split: 11 frames, 12 waves either way
+    assert r21.block == 256 and r21.push == nq.NQ_PUSH_ALWAYS and r21.stack_depth == 11 and r21.grid > 0
+    r20 = nq.resolve_config(20, 9, big)  # 10 frames: 128 lanes fit 14 waves/CU, 256 lanes 12
+    assert r20.block == 128 and r20.grid * 2 == 14 * nq.device_info(0)["cus"]
+    assert nq.resolve_config(21, 9, big, nq.Config(block=64)).block == 64
+    assert nq.resolve_config(16, 7, 10).grid == 1  # grid capped by the lanes the tasks need
+    t = nq.generate(16, 5)
+    sums = {nq.count(16, t, cfg=nq.Config(block=b)).sum for b in (0, 128, 256)}
+    assert len(sums) == 1 and 2 * sums.pop() == published["solutions"][16]
+
+
 @pytest.mark.parametrize("push", [nq.NQ_PUSH_MASKED, nq.NQ_PUSH_ALWAYS])
 def test_push_forms_exact(push, published):
     """Both DFS push forms (exec-masked and unconditional with a spare level) give the same
