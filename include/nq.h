@@ -47,8 +47,7 @@ typedef struct nq_config {
                       /* only; ignored when per-task counts are requested). nq_solve: -1 = */
                       /* automatic.  Every child keeps the exact count, order is free.      */
     int push;         /* DFS push form: NQ_PUSH_AUTO (0) takes NQ_PUSH_ALWAYS when its spare */
-                      /* stack level keeps >= 85% of the resident waves NQ_PUSH_MASKED     */
-                      /* gets, else NQ_PUSH_MASKED.  Both                                   */
+                      /* stack level costs no resident waves, else NQ_PUSH_MASKED.  Both    */
                       /* give identical counts.                                              */
 } nq_config;
 

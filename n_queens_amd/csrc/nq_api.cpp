@@ -139,8 +139,8 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
         return fail(NQ_ERR_DEPTH, buf);
     }
     // Resident workgroups per CU of a workgroup size for both push forms (capped by the
-    // waves_per_cu target); the unconditional push is taken unless its spare stack level costs
-    // over 15% of them.
+    // waves_per_cu target); the unconditional push is taken when its spare stack level costs
+    // none of them.
     auto fit = [&](int block, int *push, int *bpc) -> int {
         int occ[2] = {0, 0};
         NQ_HIP(dfs_occupancy(block, c.stack_depth, nodes, per_task, false, &occ[0]));
@@ -153,8 +153,8 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
     };
     // Automatic workgroup size: 256 lanes (one wave of each workgroup per SIMD) when that keeps
     // the resident waves 128 lanes get, else 128.  With equal waves 256 is about 0.5% faster
-    // (N=20 and N=21, 6 rows, 12 waves/CU); where 128 fits an odd number of workgroups, 256
-    // would lose waves (N=20 after a 3-row split: 14 against 12).
+    // (N=20 and N=21, 6 rows, 12 waves/CU, unconditional push); where 128 fits an odd number
+    // of workgroups, 256 would lose waves (N=21 after a 3-row split, masked push: 14 against 12).
     const bool auto_block = c.block == 0;
     if (auto_block) c.block = 128;
     int push = 0, bpc = 0;

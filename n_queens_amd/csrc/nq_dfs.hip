@@ -21,8 +21,8 @@
 //    so lanes of one wave may work on different board sizes (batch_kernel).
 //  * Two push forms: exec-masked (only frames with candidates left are written), or an
 //    unconditional write with sp advanced by stride * min(v, 1) (no exec switch, one spare
-//    stack level); the host takes the second unless the spare level costs more than 15% of
-//    the resident waves (prefer_always_push).
+//    stack level); the host takes the second when the spare level costs no resident
+//    workgroup (prefer_always_push).
 //  * A lane whose task is exhausted claims the next one at once, in the same loop (lane-
 //    granular refill), from a global cursor; claims are aggregated per wave and handed out in
 //    chunks through a wave-private LDS range.
@@ -120,20 +120,22 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 // where lanes with sp == 0 are found and the asm statement returns for their refill.  Done
 // lanes thus idle until the end of the trip, as they would out of exec.
 //
-// Only SALU instructions read the masks the VALU compares write, and every LDS read is waited
-// for before the iteration ends, so no hazard wait states are needed.
+// The push and the pop take their lanes with v_cmpx (the compare writes exec), and exec goes
+// back to the live mask with one s_mov: exec outside those two regions is always `live`, so
+// there is no VALU -> SGPR -> SALU mask round trip on the chain.  No DPP, readlane or VMEM
+// follows an exec write, and every LDS read is waited for before the iteration ends, so no
+// hazard wait states are needed.
 #define NQ_EXTRACT                                                                            \
     "v_sub_u32 %[t], 0, v23\n\t"                                                              \
     "v_and_b32 %[p], v23, %[t]\n\t"                                                           \
     "v_xor_b32 v23, v23, %[p]\n\t"
 
-// Push only the lanes whose frame keeps untried candidates (exec-masked).
+// Push only the lanes whose frame keeps untried candidates (exec-masked by v_cmpx).
 #define NQ_PUSH_MASKED                                                                        \
-    "v_cmp_ne_u32 %[m1], 0, v23\n\t"                                                          \
-    "s_and_saveexec_b64 %[tmp], %[m1]\n\t"                                                    \
+    "v_cmpx_ne_u32_e64 %[m1], 0, v23\n\t"                                                     \
     "ds_write_b128 %[sp], v[20:23]\n\t"                                                       \
     "v_add_u32 %[sp], %[stride], %[sp]\n\t"                                                   \
-    "s_mov_b64 exec, %[tmp]\n\t"
+    "s_mov_b64 exec, %[live]\n\t"
 
 // Write every lane's frame and keep it (advance sp by stride * min(v, 1)) only where
 // candidates remain: no exec switch.  A lane whose every level is in use writes one level
@@ -144,20 +146,20 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
     "v_min_u32 %[t], 1, v23\n\t"                                                              \
     "v_mad_u32_u24 %[sp], %[t], %[stride], %[sp]\n\t"
 
+// The child's right diagonal is started first so that no VALU waits on the one before it.
 // LANE_CNT adds the completed board (vcc) to the lane's counter; WAVE_CNT adds the popcount of
 // vcc to a wave-uniform SGPR counter and sits in the shadow of the pop's LDS read.
 #define NQ_CHILD_POP(LANE_CNT, WAVE_CNT)                                                      \
-    "v_or_b32 v20, v20, %[p]\n\t"                                                             \
-    "v_add_lshl_u32 v21, v21, %[p], 1\n\t"                                                    \
     "v_or_b32 v22, v22, %[p]\n\t"                                                             \
+    "v_or_b32 v20, v20, %[p]\n\t"                                                             \
     "v_lshrrev_b32 v22, 1, v22\n\t"                                                           \
-    "v_bitop3_b32 v23, v21, v22, v20 bitop3:1\n\t"                                            \
-    "v_cmp_eq_u32 vcc, -1, v20\n\t" LANE_CNT                                                  \
-    "v_cmp_eq_u32 %[m1], 0, v23\n\t"                                                          \
-    "s_and_saveexec_b64 %[tmp], %[m1]\n\t"                                                    \
+    "v_add_lshl_u32 v21, v21, %[p], 1\n\t"                                                    \
+    "v_cmp_eq_u32 vcc, -1, v20\n\t"                                                           \
+    "v_bitop3_b32 v23, v21, v22, v20 bitop3:1\n\t" LANE_CNT                                   \
+    "v_cmpx_eq_u32_e64 %[m1], 0, v23\n\t"                                                     \
     "v_sub_u32_e64 %[sp], %[sp], %[stride] clamp\n\t"                                         \
     "ds_read_b128 v[20:23], %[sp]\n\t"                                                        \
-    "s_mov_b64 exec, %[tmp]\n\t" WAVE_CNT                                                     \
+    "s_mov_b64 exec, %[live]\n\t" WAVE_CNT                                                    \
     "s_waitcnt lgkmcnt(0)\n\t"
 
 #define NQ_CNT_LANE "v_addc_co_u32 %[cnt], vcc, 0, %[cnt], vcc\n\t"
@@ -190,7 +192,7 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
         "s_mov_b64 exec, %[save]\n\t"                                                         \
         : "+{v20}"(c), "+{v21}"(l), "+{v22}"(r), "+{v23}"(v), [sp] "+v"(sp), [cnt] "+v"(cnt),   \
           [nd] "+v"(nd), [iters] "+s"(iters), [wcnt] "+s"(wcnt), [t] "=&v"(t), [p] "=&v"(p),   \
-          [m1] "=&s"(m1), [tmp] "=&s"(tmp), [save] "=&s"(save), [bc] "=&s"(bc)                  \
+          [m1] "=&s"(m1), [save] "=&s"(save), [bc] "=&s"(bc)                                   \
         : [live] "s"(live), [stride] "s"(stride)                                              \
         : "memory", "scc", "vcc")
 
@@ -208,7 +210,7 @@ __device__ __forceinline__ void hot_loop(uint32_t &c, uint32_t &l, uint32_t &r, 
                                          uint32_t &cnt, uint32_t &wcnt, uint32_t &nd, uint64_t live,
                                          uint32_t stride) {
     uint32_t t, p, bc, iters = WAVE ? FLUSH_EVERY_WAVE : FLUSH_EVERY;
-    uint64_t m1, tmp, save;
+    uint64_t m1, save;
     if (WAVE) {
         static_assert(!WAVE || !NODES, "the node-counting variant counts per lane");
         if (APUSH)

@@ -62,11 +62,11 @@ hipError_t batch_expand(uint4 **tasks, uint64_t *n, uint32_t target, int levels,
 hipError_t launch_batch(const BatchArgs &a, hipStream_t stream);
 hipError_t batch_occupancy(int block, int stack_depth, bool apush, int *blocks_per_cu);
 // Push form for push 0 (automatic) from the resident workgroups per CU of each form (same
-// block).  At equal occupancy the unconditional push is about 9% faster (N=20 and N=21, 6 rows),
-// and with 12 waves/CU it still edges out the masked push's 14; so it is taken while its spare
-// stack level keeps at least 85% of the masked push's workgroups.
+// block).  At equal occupancy the unconditional push is about 7% faster (N=20 and N=21, 6 rows),
+// but the masked push with 14 waves/CU beats it with 12 by about 1%; so it is taken only when
+// its spare stack level costs no workgroup.
 inline bool prefer_always_push(int occ_masked, int occ_always) {
-    return occ_always > 0 && 100 * occ_always >= 85 * occ_masked;
+    return occ_always > 0 && occ_always >= occ_masked;
 }
 
 // Push form and resident workgroups per CU for a batched launch: push 0 automatic

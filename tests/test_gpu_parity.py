@@ -193,10 +193,14 @@ def test_automatic_block_and_resolved_launch(published):
     """Block 0 resolves to 256 when that keeps the resident waves of 128, else 128; explicit
     blocks are kept; counts do not depend on it."""
     big = 1 << 40
-    r21 = nq.resolve_config(21, 9, big)  # N=21 after a 3-row split: 11 frames, 12 waves either way
-    assert r21.block == 256 and r21.push == nq.NQ_PUSH_ALWAYS and r21.stack_depth == 11 and r21.grid > 0
-    r20 = nq.resolve_config(20, 9, big)  # 10 frames: 128 lanes fit 14 waves/CU, 256 lanes 12
-    assert r20.block == 128 and r20.grid * 2 == 14 * nq.device_info(0)["cus"]
+    cus = nq.device_info(0)["cus"]
+    # N=21 after a 3-row split, 11 frames: the masked push keeps 14 waves/CU with 128 lanes (the
+    # spare level of the unconditional push, or 256 lanes, would leave 12).
+    r21 = nq.resolve_config(21, 9, big)
+    assert (r21.block, r21.push, r21.stack_depth) == (128, nq.NQ_PUSH_MASKED, 11) and r21.grid * 2 == 14 * cus
+    # 12 frames: 12 waves/CU with either push form or block, so the unconditional push and 256 lanes.
+    r12 = nq.resolve_config(21, 8, big)
+    assert (r12.block, r12.push) == (256, nq.NQ_PUSH_ALWAYS) and r12.grid * 4 == 12 * cus
     assert nq.resolve_config(21, 9, big, nq.Config(block=64)).block == 64
     assert nq.resolve_config(16, 7, 10).grid == 1  # grid capped by the lanes the tasks need
     t = nq.generate(16, 5)
