@@ -40,11 +40,11 @@ namespace nq {
 
 namespace {
 
-// Hot-loop trips (8 iterations each) between folds of the 32-bit per-lane counters: at most
-// 8 * 2^28 = 2^31 increments between folds.
-constexpr uint32_t FLUSH_EVERY = 1u << 28;
+// Hot-loop trips (16 iterations each) between folds of the 32-bit per-lane counters: at most
+// 16 * 2^27 = 2^31 increments between folds.
+constexpr uint32_t FLUSH_EVERY = 1u << 27;
 // Trips between folds of the wave-uniform 32-bit counter (64 lanes add to it).
-constexpr uint32_t FLUSH_EVERY_WAVE = 1u << 22;
+constexpr uint32_t FLUSH_EVERY_WAVE = 1u << 21;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
@@ -116,7 +116,7 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 // lane's level 0 lies less than one stride above it, so the saturating subtract (v_sub_u32
 // with clamp) sends the pop of an empty stack to address 0.  A lane on the dead frame places
 // nothing (p = 0), its child is dead again (~(c | l<<1 | r>>1) == 0, c != ~0: nothing is
-// counted) and it pops the dead frame again, until the end of the trip (eight iterations),
+// counted) and it pops the dead frame again, until the end of the trip (16 iterations),
 // where lanes with sp == 0 are found and the asm statement returns for their refill.  Done
 // lanes thus idle until the end of the trip, as they would out of exec.
 //
@@ -172,7 +172,7 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 #define NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
     NQ_ITER(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)
 
-// The whole hot loop: eight iterations per trip, leaving after a trip that ends with some lane
+// The whole hot loop: 16 iterations per trip, leaving after a trip that ends with some lane
 // on the dead frame (sp == 0), or after [iters] trips (the caller folds the 32-bit counters
 // then).  [nd] counts expansions (node-counting variant only).
 #define NQ_HOT_ASM(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)                                      \
@@ -180,6 +180,8 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
         "s_mov_b64 %[save], exec\n\t"                                                         \
         "s_mov_b64 exec, %[live]\n"                                                           \
         "nq_hot_%=:\n\t" NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)                        \
+        NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
+        NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
         NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
         NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)                                         \
         "v_cmp_eq_u32 %[m1], 0, %[sp]\n\t"                                                    \
@@ -204,7 +206,7 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 // Runs the hot loop for the lanes in `live` until one of them finishes its task (or the fold
 // interval ends).  APUSH selects the unconditional push (the stack has a spare level).  WAVE
 // counts completed boards per wave into wcnt (SALU) instead of per lane into cnt; its fold
-// interval is 2^22 trips, at most 2^22 * 8 * 64 = 2^31 boards.
+// interval is 2^21 trips, at most 2^21 * 16 * 64 = 2^31 boards.
 template <bool NODES, bool APUSH, bool WAVE>
 __device__ __forceinline__ void hot_loop(uint32_t &c, uint32_t &l, uint32_t &r, uint32_t &v, uint32_t &sp,
                                          uint32_t &cnt, uint32_t &wcnt, uint32_t &nd, uint64_t live,

@@ -1,7 +1,7 @@
 /* CPU model of the DFS loop of n_queens_amd/csrc/nq_dfs.hip (NQ_HOT_ASM and its callers), test
  * infrastructure only.
  *
- * One workgroup of B lanes runs the kernel's per-lane iteration in lockstep trips of 8
+ * One workgroup of B lanes runs the kernel's per-lane iteration in lockstep trips of TRIP (16)
  * iterations over an LDS byte array with the kernel's layout (lds_layout(): a dead frame at
  * address 0, then stack[levels][B] of 16-B frames).  Between trips, lanes whose sp is 0 (parked
  * on the dead frame) take the next task, as the kernel's refill does.  It checks:
@@ -24,6 +24,8 @@
 typedef struct {
     uint32_t c, l, r, v;
 } frame;
+
+#define TRIP 16 /* iterations per trip of the hot loop (NQ_HOT_ASM) */
 
 #ifdef PLANT_FULL_DEAD
 static const frame DEAD = {0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u};
@@ -123,12 +125,12 @@ static int run(int N, int rows, int B, int apush) {
             any |= has[i];
         }
         if (!any) break;
-        for (;;) { /* trips of 8 iterations until some lane is parked */
+        for (;;) { /* trips of TRIP iterations until some lane is parked */
             if (++trips > 50000000L) {
                 fprintf(stderr, "N=%d rows=%d B=%d apush=%d: no termination\n", N, rows, B, apush);
                 return 1;
             }
-            for (int it = 0; it < 8; it++)
+            for (int it = 0; it < TRIP; it++)
                 for (int i = 0; i < B; i++) {
                     if (!has[i]) continue;
                     frame *x = &f[i];
