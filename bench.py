@@ -33,7 +33,7 @@ LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
 # HBM bytes per N=21 rows-6 launch of the DFS kernel (3 rows split on the device), from two
 # rocprofv3 --pmc passes scaled by a 256 MiB copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000):
 # profiles/dfs_kernel_n21_rows6_traffic.md.
-TRAFFIC_N21_ROWS6 = (25398141 + 3278846) * 1024
+TRAFFIC_N21_ROWS6 = (25397496 + 3278870) * 1024
 
 
 def log(msg: str) -> None:
