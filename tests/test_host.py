@@ -209,3 +209,26 @@ def test_tuner_occupancy_candidates():
     assert [c.waves_per_cu for c in occ[:4]] == [4, 8, 12, 16]
     assert [c.stack_depth for c in occ[4:]] == [13, 15, 17]
     assert all(c.split == 2 and c.block == 128 for c in occ)
+
+
+def test_default_and_solve_configs_leave_block_automatic():
+    """nq_config_default / nq_solve_config need no device: workgroup size 0 (automatic, chosen by
+    occupancy at launch), the smallest legal stack, chunk 16, automatic push, and the split rule
+    clamp(N - rows - (N+4)/2, 0, 3)."""
+    c = nq.config_default(21, 9)
+    assert (c.block, c.stack_depth, c.chunk, c.push, c.split) == (0, 11, 16, nq.NQ_PUSH_AUTO, 0)
+    for N, rows, split in ((21, 6, 3), (20, 6, 2), (18, 6, 1), (16, 4, 2), (13, 5, 0), (28, 7, 3)):
+        s = nq.solve_config(N, rows)
+        assert s.split == split and s.block == 0 and s.stack_depth == N - 1 - min(N, rows + split), (N, rows)
+
+
+def test_config_resolve_needs_a_device():
+    r = subprocess.run([sys.executable, "-c", """
+import n_queens_amd as nq
+try:
+    nq.resolve_config(21, 9, 1 << 30)
+except nq.NQError as e:
+    print("status", e.status)
+"""], capture_output=True, text=True, env=NO_GPU, cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+        timeout=120)
+    assert f"status {nq.NQ_ERR_NO_DEVICE}" in r.stdout, r.stdout + r.stderr
