@@ -30,10 +30,10 @@ METRIC = "N=21 wall s + DFS nodes/s on 1 GPU; batched-mix instances/s vs per-lau
 OPS_PER_NODE = 11          # p=v&-v (2), v^=p (1), c|=p (1), l=(l|p)<<1 (2), r=(r|p)>>1 (2), v'=last&~(c|l|r) (3)
 PEAK_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
 LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
-# HBM bytes per N=21 rows-6 launch of the DFS kernel (3 rows split on the device), from two
+# HBM bytes per N=21 rows-6 launch of the DFS kernel (2 rows split on the device), from two
 # rocprofv3 --pmc passes scaled by a 256 MiB copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000):
 # profiles/dfs_kernel_n21_rows6_traffic.md.
-TRAFFIC_N21_ROWS6 = (25397496 + 3278870) * 1024
+TRAFFIC_N21_ROWS6 = (4136125 + 533930) * 1024
 
 
 def log(msg: str) -> None:

@@ -84,8 +84,8 @@ int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t 
 int nq_config_default(int N, int d0, nq_config *cfg);
 
 /* The configuration nq_solve(N, rows, NULL, ...) uses: defaults for the tasks after the
- * automatic device split of clamp(N - rows - (N+4)/2, 0, 3) rows (the DFS starts (N+4)/2 rows
- * above the last row). */
+ * automatic device split of clamp(N - rows - R, 0, 3) rows, so that the DFS starts R rows above
+ * the last row: R = (N+4)/2 up to N = 20, R = 13 from N = 21 on (12 LDS frames). */
 int nq_solve_config(int N, int rows, nq_config *cfg);
 
 /* The launch configuration nq_count takes on the current device for ntasks tasks whose

@@ -296,14 +296,18 @@ static int min_depth(int N, const uint32_t *tasks, uint64_t ntasks) {
     return d0;
 }
 
-// Rows nq_solve expands on the device (0..3): the DFS then starts (N + 4) / 2 rows above the
-// last row.  Shallower starts leave too few, too long tasks (drain tail, and more LDS frames per
-// lane, i.e. fewer waves per CU); deeper starts make tasks so short that claiming and loading
-// them dominates.  Measured best on one MI355X (kernel + expansion time): N=21 rows 6 -> 3
-// (8.34 s; 2: 8.42 s, 0: 10.31 s), N=20/6 -> 2 (0.98 s), N=19/6 -> 2, N=18/6 -> 1, N=17/5 -> 2,
-// N=16/4 -> 2; the rule gives each of these.
+// Rows nq_solve expands on the device (0..3): the DFS then starts dfs_rows(N) rows above the
+// last row, i.e. with dfs_rows(N) - 1 LDS frames.  Measured on one MI355X (split + DFS, final
+// loop): N=16/4 -> 2 (1.1 ms), N=17/5 -> 1 or 2 (4.0 / 4.1 ms), N=18/6 -> 1 (18.0 ms; 0: 25.0,
+// 2: 31.8), N=19/6 -> 2 (112.5 ms; 1: 118.2), N=20/6 -> 2 (881.6 ms; 1: 894.4, 3: 890.3),
+// N=21/6 -> 2 (7.36 s; 3: 7.47 s), N=22/6 -> 3 (65.5 s; 2: 69.8 s).  Up to N=20 shorter tasks
+// (the drain tail) decide, and (N + 4) / 2 rows fits.  From N=21 on, 12 frames win: the
+// unconditional push then keeps 12 waves/CU in 256-lane workgroups, while 13 frames leave the
+// masked push at 12 waves (about 7% slower) and 11 frames make 6 times more, shorter tasks.
+static int dfs_rows(int N) { return N <= 20 ? (N + 4) / 2 : 13; }
+
 static int auto_split(int N, int rows) {
-    return std::max(0, std::min(3, N - rows - (N + 4) / 2));
+    return std::max(0, std::min(3, N - rows - dfs_rows(N)));
 }
 
 
