@@ -10,9 +10,10 @@
 //    on every active lane; there is no per-iteration reload of the top frame.
 //  * Before a frame is replaced by its child it is pushed to LDS only if it still has
 //    untried candidates (tail-call elimination); a lane whose frame runs empty pops.
-//    Frames are 16 B {c, l, r, v} at stack[level][threadIdx.x]: lanes of a wave touch 64
-//    consecutive 16-B slots, so ds_write_b128 / ds_read_b128 are bank-conflict free whatever
-//    depth each lane is at.
+//    Frames are 16 B {c, l, r, v} at stack[level][threadIdx.x], behind a dead frame at LDS
+//    address 0 that the pop of an empty stack lands on (no stack-empty test; see the loop):
+//    lanes of a wave touch 64 consecutive 16-B slots, so ds_write_b128 / ds_read_b128 of real
+//    frames are bank-conflict free whatever depth each lane is at.
 //  * The columns word c has every bit from N up set: the candidates of a frame are then
 //    ~(l | r | c) (one v_bitop3, no board mask) and a completed board is c == ~0.  It is
 //    counted with a compare and either an add-with-carry into a per-lane counter (per-task
@@ -23,9 +24,9 @@
 //    unconditional write with sp advanced by stride * min(v, 1) (no exec switch, one spare
 //    stack level); the host takes the second when the spare level costs no resident
 //    workgroup (prefer_always_push).
-//  * A lane whose task is exhausted claims the next one at once, in the same loop (lane-
-//    granular refill), from a global cursor; claims are aggregated per wave and handed out in
-//    chunks through a wave-private LDS range.
+//  * A lane whose task is exhausted waits on the dead frame until the end of its 16-iteration
+//    trip, then claims the next task (lane-granular refill) from a global cursor; claims are
+//    aggregated per wave and handed out in chunks through a wave-private LDS range.
 //  * Per-lane counts are 32-bit (a lane adds at most 1 per iteration) and folded into 64-bit
 //    every 2^31 iterations and at task end; the 64-bit sums are reduced per wave with DPP
 //    shuffles, per workgroup through LDS, and added with one atomic per workgroup.
@@ -305,7 +306,7 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
     bool has = true;      // lane still has (or may claim) work
 
     for (;;) {
-        // Lanes whose task is exhausted (v == 0 at the stack bottom) claim the next one.
+        // Lanes whose task is exhausted (parked on the dead frame: v == 0) claim the next one.
         if (has && v == 0) {
             const uint64_t done = tsum;
             acc += done;
