@@ -213,10 +213,9 @@ def main() -> int:
     d_result = torch.zeros(3, dtype=torch.int64, device=dev)
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.synchronize()
-    # The device API's stack_depth bounds the tasks as uploaded (depth `rows`); the library takes
-    # the split rows off it for the DFS after the split.
+    # nq_solve's configuration: stack_depth bounds the tasks as uploaded (depth `rows`); the
+    # library takes the split rows off it for the DFS after the split.
     cfg = nq.solve_config(N, rows)
-    cfg.stack_depth = N - 1 - rows
     if args.split >= 0:
         cfg = nq.config_default(N, rows)
         cfg.split = args.split

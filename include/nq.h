@@ -83,9 +83,11 @@ int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t 
  * that check (src/main.cpp:16-19). */
 int nq_config_default(int N, int d0, nq_config *cfg);
 
-/* The configuration nq_solve(N, rows, NULL, ...) uses: defaults for the tasks after the
- * automatic device split of clamp(N - rows - R, 0, 3) rows, so that the DFS starts R rows above
- * the last row: R = (N+4)/2 up to N = 20, R = 13 from N = 21 on (12 LDS frames). */
+/* The configuration nq_solve(N, rows, NULL, ...) uses, in the form nq_solve and nq_count take
+ * it back: stack_depth = N - 1 - rows bounds the depth-`rows` tasks (the count takes the split
+ * rows off it), and split = the automatic device split of clamp(N - rows - R, 0, 3) rows, so
+ * that the DFS starts R rows above the last row: R = (N+4)/2 up to N = 20, R = 13 from N = 21
+ * on (12 LDS frames).  nq_solve(N, rows, &cfg, ...) with this cfg is nq_solve(N, rows, NULL). */
 int nq_solve_config(int N, int rows, nq_config *cfg);
 
 /* The launch configuration nq_count takes on the current device for ntasks tasks whose

@@ -90,13 +90,15 @@ class _ResumeState(ctypes.Structure):
 
 @dataclass
 class Config:
-    """Launch configuration (0 = library default).  See nq_config in include/nq.h."""
+    """Launch configuration.  See nq_config in include/nq.h: 0 = library default for every
+    field except split, where 0 means no device split and -1 (the default here) the library's
+    automatic choice, as solve(N, rows) without a Config makes it."""
     block: int = 0
     stack_depth: int = 0
     waves_per_cu: int = 0
     chunk: int = 0
     grid: int = 0
-    split: int = 0
+    split: int = -1
     push: int = 0
 
     def _c(self) -> _Config:

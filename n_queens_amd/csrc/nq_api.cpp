@@ -608,9 +608,10 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
 int nq_solve_config(int N, int rows, nq_config *cfg) {
     if (!cfg) return fail(NQ_ERR_ARG, "cfg is NULL");
     if (N < 1 || N > MAX_N || rows < 1 || rows > N) return fail(NQ_ERR_ARG, "N must be 1..28 and rows 1..N");
-    const int split = auto_split(N, rows);
-    nq_config_default(N, std::min(N, rows + split), cfg);
-    cfg->split = split;
+    // stack_depth bounds the depth-`rows` tasks, as nq_solve and nq_count read it (they take the
+    // split rows off it for the DFS after the split).
+    nq_config_default(N, rows, cfg);
+    cfg->split = auto_split(N, rows);
     return NQ_OK;
 }
 

@@ -217,6 +217,8 @@ int nq_batch_plan_create(const nq_instance *inst, int n, const nq_config *cfg, n
     if (!out) return fail(NQ_ERR_ARG, "out is NULL");
     *out = nullptr;
     if (n < 0 || (n > 0 && !inst)) return fail(NQ_ERR_ARG, "bad instance array");
+    // A batched task carries its instance index in tag bits 6..31.
+    if (n > (1 << 26)) return fail(NQ_ERR_ARG, "at most 2^26 instances per batch");
     const double t0 = now_ms();
     nq_config c;
     nq_config_default(MAX_N, 0, &c);

@@ -17,14 +17,7 @@
 
 #include "n_queens.h"
 #include "nq.h"
-
-namespace nq {
-void log_line(const char *fmt, ...);
-}
-
-static float time_diff_ms(const timeval &s, const timeval &e) {
-    return (e.tv_sec - s.tv_sec) * 1000.0 + (e.tv_usec - s.tv_usec) / 1000.0;  // src/utils.cpp:10-12
-}
+#include "utils.h"
 
 int main(int argc, char *argv[]) {
     int config = 2;
@@ -90,13 +83,13 @@ int main(int argc, char *argv[]) {
         gettimeofday(&end, nullptr);
         printf("parallel %d queens result %lld, calc time: [%.2fms]\n", N, cpu_sum, time_diff_ms(start, end));
     }
-    nq::log_line("===============================================================\n");
+    print_with_time("===============================================================\n");
     gettimeofday(&start, nullptr);
     long long sum = -1;
     if (checkpoint) {
         nq_resume_state st;
         if (nq_solve_resumable(N, rows, nullptr, checkpoint, range_tasks, 0, &st) == NQ_OK) {
-            nq::log_line("checkpoint %s: %d ranges counted in this run, tasks %llu of %llu done\n", checkpoint,
+            print_with_time("checkpoint %s: %d ranges counted in this run, tasks %llu of %llu done\n", checkpoint,
                          st.ranges, static_cast<unsigned long long>(st.next), static_cast<unsigned long long>(st.ntasks));
             sum = static_cast<long long>(st.total);
         } else {
@@ -106,7 +99,7 @@ int main(int argc, char *argv[]) {
         sum = cuda_n_queens(N, rows);
     }
     gettimeofday(&end, nullptr);
-    nq::log_line("cuda %d queens result %lld, calc time: [%.2fms]\n", N, sum, time_diff_ms(start, end));
-    nq::log_line("===============================================================\n");
+    print_with_time("cuda %d queens result %lld, calc time: [%.2fms]\n", N, sum, time_diff_ms(start, end));
+    print_with_time("===============================================================\n");
     return 0;
 }

@@ -1,5 +1,7 @@
-// C++ drop-in entry points of include/n_queens.h, on top of the C ABI (nq.h).
+// C++ drop-in entry points of include/n_queens.h and include/utils.h, on top of the C ABI
+// (nq.h).
 #include "n_queens.h"
+#include "utils.h"
 
 #include <stdarg.h>
 #include <stdint.h>
@@ -12,11 +14,13 @@
 #include "nq.h"
 #include "nq_internal.h"
 
-namespace nq {
+// src/utils.cpp:10-12: milliseconds between two gettimeofday stamps.
+float time_diff_ms(struct timeval &start, struct timeval &end) {
+    return static_cast<float>((end.tv_sec - start.tv_sec) * 1000.0 + (end.tv_usec - start.tv_usec) / 1000.0);
+}
 
-// "[YYYY-mm-dd HH:MM:SS.mmm] " + message, the reference's log line format
-// (print_with_time, src/utils.cpp:14-30).
-void log_line(const char *fmt, ...) {
+// src/utils.cpp:14-30: the message behind a "[YYYY-mm-dd HH:MM:SS.mmm] " local-time stamp.
+void print_with_time(const char *format, ...) {
     timeval tv;
     gettimeofday(&tv, nullptr);
     time_t secs = tv.tv_sec;
@@ -26,12 +30,10 @@ void log_line(const char *fmt, ...) {
     strftime(stamp, sizeof stamp, "[%Y-%m-%d %H:%M:%S.", &t);
     fprintf(stdout, "%s%03d] ", stamp, static_cast<int>(tv.tv_usec / 1000));
     va_list ap;
-    va_start(ap, fmt);
-    vfprintf(stdout, fmt, ap);
+    va_start(ap, format);
+    vfprintf(stdout, format, ap);
     va_end(ap);
 }
-
-}  // namespace nq
 
 static void append(vector<int> &tot, const std::vector<uint32_t> &w) {
     tot.reserve(tot.size() + w.size());
@@ -53,8 +55,8 @@ void partial_n_queens_for_odd(int N, int cur, int left, int right, vector<int> &
 }
 
 static void log_generated(void *, uint64_t ntasks, double ms) {
-    nq::log_line("Use %.2fms to generate %lld subproblems!\n", ms, static_cast<long long>(ntasks));
-    nq::log_line("gpu [%d] start job, with %lld(%.2f) subproblems.\n", 0, static_cast<long long>(ntasks), 1.0);
+    print_with_time("Use %.2fms to generate %lld subproblems!\n", ms, static_cast<long long>(ntasks));
+    print_with_time("gpu [%d] start job, with %lld(%.2f) subproblems.\n", 0, static_cast<long long>(ntasks), 1.0);
 }
 
 long long cuda_n_queens(int N, int rows) {
@@ -63,7 +65,7 @@ long long cuda_n_queens(int N, int rows) {
         const double t0 = nq::now_ms();
         std::vector<uint32_t> tasks;
         nq::generate_tasks(N, rows, tasks);
-        nq::log_line("Use %.2fms to generate %lld subproblems!\n", nq::now_ms() - t0,
+        print_with_time("Use %.2fms to generate %lld subproblems!\n", nq::now_ms() - t0,
                      static_cast<long long>(tasks.size() / 3));
         printf("Failed to find any gpu!\n");  // src/n_queens_cuda.cu:105-108
         return -1;
@@ -76,6 +78,6 @@ long long cuda_n_queens(int N, int rows) {
         fprintf(stderr, "cuda_n_queens: %s\n", nq_last_error());
         return -1;
     }
-    nq::log_line("gpu [%d] finish job.\n", 0);
+    print_with_time("gpu [%d] finish job.\n", 0);
     return static_cast<long long>(total);
 }

@@ -5,6 +5,10 @@
 //
 //   n_queens(N, cur, left, right, sum)  adds the completions of the prefix (cur, left, right)
 //                                       to sum (src/n_queens.cpp:8-21; sum accumulates)
+//   n_queens_iterative(N, cur, left, right, sum)
+//                                       the reference's explicit-stack variant
+//                                       (src/n_queens.cpp:24-72): the same sum, except that a
+//                                       prefix already holding N-1 or N queens adds nothing
 //   serial_n_queens(N, rows)            generator + every task + x2 (src/n_queens.cpp:115-135)
 //   parallel_n_queens(N, rows)          the same over 32 threads (src/n_queens.cpp:137-163)
 #include <stdint.h>
@@ -78,6 +82,18 @@ long long fold_tasks(int N, int rows, int threads) {
 void n_queens(int N, int cur, int left, int right, long long &sum) {
     sum += static_cast<long long>(
         count_prefix(N, static_cast<uint32_t>(cur), static_cast<uint32_t>(left), static_cast<uint32_t>(right)));
+}
+
+// The reference's loop counts a board when a placement leaves one free column that is still a
+// candidate (src/n_queens.cpp:62-66), so the placement that fills the board is never reached
+// from a prefix of N-1 queens, and a full board has no candidate to start from (:29-31).  Any
+// other prefix gets its exact completion count.  Its 16-frame stack (int stack[64], :26)
+// overflows more than 16 rows above the last row; the count here has no such limit.
+void n_queens_iterative(int N, int cur, int left, int right, long long &sum) {
+    const uint32_t last = (1u << N) - 1u;
+    const uint32_t c = static_cast<uint32_t>(cur);
+    if (__builtin_popcount(c & last) + 1 >= N) return;
+    sum += static_cast<long long>(count_prefix(N, c, static_cast<uint32_t>(left), static_cast<uint32_t>(right)));
 }
 
 long long serial_n_queens(int N, int rows) { return fold_tasks(N, rows, 1); }

@@ -43,7 +43,7 @@ struct LaunchArgs {
 };
 
 struct BatchArgs {
-    const uint4 *tasks;  // {cur, left, right, inst << 5 | N}
+    const uint4 *tasks;  // {cur, left, right, inst << 6 | twice << 5 | N}
     uint64_t ntasks;
     uint64_t *counts;    // per instance, zeroed by the launch
     uint64_t ninst;

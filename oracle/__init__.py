@@ -15,6 +15,8 @@ Parity status of each oracle function (how it is pinned, see ``tests/test_oracle
                        exact task lists vs ``_ref`` (N=4..16), golden fixtures, invariants.
 * ``count_subtree``  — totals vs OEIS A000170 (README.md:16-25, logs), permutation brute force
                        N<=9, per-task counts vs ``_ref`` and golden fixtures.
+* ``count_iterative``— the reference's iterative variant: equal to ``_ref`` on random
+                       prefixes, to ``count_subtree`` below N-1 queens, 0 at N-1 and N queens.
 * ``count_nodes``    — brute-force recursion with an independent definition (placements at
                        depths rows+1..N-1), closed form for the full tree on tiny N.
 * ``serial/parallel``— A000170 and ``_ref``.
@@ -53,7 +55,7 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(_LIB)
         L.oracle_gen_tasks.restype = ctypes.c_uint64
         L.oracle_gen_tasks.argtypes = [ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_uint64]
-        for name in ("oracle_count_subtree", "oracle_count_nodes"):
+        for name in ("oracle_count_subtree", "oracle_count_nodes", "oracle_count_iterative"):
             f = getattr(L, name)
             f.restype = ctypes.c_uint64
             f.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
@@ -81,6 +83,8 @@ def ref():
         R.ref_gen_tasks.argtypes = [ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_uint64]
         R.ref_count_subtree.restype = ctypes.c_uint64
         R.ref_count_subtree.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        R.ref_count_iterative.restype = ctypes.c_uint64
+        R.ref_count_iterative.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         R.ref_serial_n_queens.restype = ctypes.c_uint64
         R.ref_serial_n_queens.argtypes = [ctypes.c_int, ctypes.c_int]
         R.ref_parallel_n_queens.restype = ctypes.c_uint64
@@ -112,6 +116,10 @@ def ref_gen_tasks(N: int, rows: int) -> np.ndarray:
 
 def count_subtree(N: int, cur: int, left: int, right: int) -> int:
     return int(lib().oracle_count_subtree(N, cur & 0xFFFFFFFF, left & 0xFFFFFFFF, right & 0xFFFFFFFF))
+
+
+def count_iterative(N: int, cur: int, left: int, right: int) -> int:
+    return int(lib().oracle_count_iterative(N, cur & 0xFFFFFFFF, left & 0xFFFFFFFF, right & 0xFFFFFFFF))
 
 
 def count_nodes(N: int, cur: int, left: int, right: int) -> int:

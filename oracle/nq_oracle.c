@@ -33,6 +33,53 @@ uint64_t oracle_count_subtree(int N, uint32_t cur, uint32_t left, uint32_t right
     return s;
 }
 
+/* ---- iterative counter: src/n_queens.cpp:24-72 ---------------------------------------
+ * The reference's explicit-stack variant.  It counts a board when a placement leaves exactly
+ * one free column that is still a candidate (:62-66), so a prefix that already holds N-1 or
+ * N queens adds nothing (its first placement fills the board and finds no candidate, :56-60),
+ * unlike n_queens above, which counts a full board as 1.  The reference's stack holds 16
+ * frames (int stack[64], :26); this restatement's holds 32, so it also covers the deeper
+ * prefixes on which the reference would overflow. */
+uint64_t oracle_count_iterative(int N, uint32_t cur, uint32_t left, uint32_t right) {
+    uint32_t last = board_mask(N);
+    uint32_t st[4 * 33];
+    int top = 0;
+    uint64_t sum = 0;
+    uint32_t valid = last & ~(cur | left | right); /* :29 */
+    if (valid == 0) return 0;                      /* :31 */
+    st[top++] = cur;
+    st[top++] = left;
+    st[top++] = right;
+    st[top++] = valid;
+    while (top != 0) { /* :38 */
+        valid = st[top - 1];
+        right = st[top - 2];
+        left = st[top - 3];
+        cur = st[top - 4];
+        uint32_t p = valid & (0u - valid); /* :44-45 */
+        valid -= p;
+        if (valid == 0) /* :47-51 */
+            top -= 4;
+        else
+            st[top - 1] = valid;
+        cur |= p; /* :53-56 */
+        left = (left | p) << 1;
+        right = (right | p) >> 1;
+        valid = last & ~(cur | left | right);
+        if (valid == 0) continue; /* :58-60 */
+        p = cur ^ last;           /* :62-66 */
+        if ((p & (p - 1)) == 0) {
+            sum++;
+            continue;
+        }
+        st[top++] = cur; /* :68-71 */
+        st[top++] = left;
+        st[top++] = right;
+        st[top++] = valid;
+    }
+    return sum;
+}
+
 /* ---- kernel node count: src/n_queens_cuda.cu:39-72 ------------------------------------
  * Every loop iteration of the reference kernel places one queen at depth popc(cur)+1; a
  * placement that reaches depth N-1 ends the branch and adds popc(valid) (the last row), so

@@ -29,6 +29,12 @@ uint64_t oracle_gen_tasks(int N, int rows, uint32_t *out, uint64_t cap);
  * columns. */
 uint64_t oracle_count_subtree(int N, uint32_t cur, uint32_t left, uint32_t right);
 
+/* The reference's n_queens_iterative (src/n_queens.cpp:24-72): completions of one prefix,
+ * except that a prefix already holding N-1 or N queens adds nothing (the reference's own
+ * behaviour).  The reference's 16-frame stack overflows for prefixes more than 16 rows above
+ * the last; this one holds 32 frames. */
+uint64_t oracle_count_iterative(int N, uint32_t cur, uint32_t left, uint32_t right);
+
 /* DFS nodes below one prefix in the GPU kernel's sense (src/n_queens_cuda.cu:39-72): one node
  * per queen placed at depths popcount(cur)+1 .. N-1 (the last row is counted by popcount and
  * is not a node). */

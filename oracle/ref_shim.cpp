@@ -33,6 +33,13 @@ uint64_t ref_count_subtree(int N, uint32_t cur, uint32_t left, uint32_t right) {
     return (uint64_t)s;
 }
 
+// n_queens_iterative (src/n_queens.cpp:24-72), also accumulating into sum.
+uint64_t ref_count_iterative(int N, uint32_t cur, uint32_t left, uint32_t right) {
+    long long s = 0;
+    n_queens_iterative(N, (int)cur, (int)left, (int)right, s);
+    return (uint64_t)s;
+}
+
 uint64_t ref_serial_n_queens(int N, int rows) { return (uint64_t)serial_n_queens(N, rows); }
 
 uint64_t ref_parallel_n_queens(int N, int rows) { return (uint64_t)parallel_n_queens(N, rows); }

@@ -29,8 +29,12 @@ def test_library_exports_every_declared_symbol():
     for name in decl:
         assert hasattr(L, name), name
     out = subprocess.run(["nm", "-D", "--defined-only", nq.LIB_PATH], capture_output=True, text=True).stdout
+    # Every function of the reference header (src/n_queens.h:7-19) and of src/utils.h:5-7, with
+    # the reference's C++ signatures.
     for cxx in ("_Z13cuda_n_queensii", "_Z16partial_n_queensiiiiRSt6vectorIiSaIiEEi",
-                "_Z24partial_n_queens_for_oddiiiiRSt6vectorIiSaIiEEi"):
+                "_Z24partial_n_queens_for_oddiiiiRSt6vectorIiSaIiEEi", "_Z8n_queensiiiiRx",
+                "_Z18n_queens_iterativeiiiiRx", "_Z15serial_n_queensii", "_Z17parallel_n_queensii",
+                "_Z12time_diff_msR7timevalS0_", "_Z15print_with_timePKcz"):
         assert cxx in out, cxx
 
 
@@ -106,6 +110,61 @@ def test_cpp_dropin_cpu_counters(tmp_path, published):
 
 
 NO_GPU = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+
+
+def test_cpp_dropin_iterative_counter(tmp_path):
+    """n_queens_iterative (src/n_queens.cpp:24-72) against the oracle's restatement, on seeded
+    prefixes of every depth: it accumulates into sum, counts the completions below N-1 queens
+    and adds nothing for N-1 queens or a full board, as the reference's loop does."""
+    rng = np.random.default_rng(0x17E6)
+    cases = []
+    while len(cases) < 200:
+        N = int(rng.integers(4, 14))
+        k = int(rng.integers(0, N + 1))
+        cols = []
+        for _ in range(k):
+            free = [c for c in range(N) if oracle.encode_prefix(N, [*cols, c]) is not None]
+            if not free:
+                break
+            cols.append(int(free[rng.integers(len(free))]))
+        if len(cols) == k:
+            cases.append((N, *oracle.encode_prefix(N, cols)))
+    src = tmp_path / "iter.cpp"
+    src.write_text(
+        '#include "n_queens.h"\n#include <cstdio>\n#include <cstdlib>\n'
+        "int main(int argc, char **argv){ for (int i = 1; i + 3 < argc; i += 4) { long long s = 7;\n"
+        " n_queens_iterative(atoi(argv[i]), (int)strtoul(argv[i+1],0,10), (int)strtoul(argv[i+2],0,10),\n"
+        ' (int)strtoul(argv[i+3],0,10), s); printf("%lld\\n", s - 7); } return 0; }\n')
+    exe = tmp_path / "iter"
+    libdir = os.path.dirname(nq.LIB_PATH)
+    subprocess.check_call(["g++", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                           "-L", libdir, "-lnq_hip", f"-Wl,-rpath,{libdir}"])
+    args = [str(x) for case in cases for x in case]
+    got = [int(x) for x in subprocess.check_output([str(exe), *args], env=NO_GPU).split()]
+    assert got == [oracle.count_iterative(*case) for case in cases]
+    assert any(g == 0 and bin(c).count("1") >= N - 1 for g, (N, c, _, _) in zip(got, cases))
+
+
+REF_SRC = "/root/reference/src"
+
+
+@pytest.mark.skipif(not os.path.isfile(os.path.join(REF_SRC, "main.cpp")), reason="no reference checkout")
+def test_reference_main_links_against_the_library(tmp_path):
+    """The reference's own src/main.cpp, compiled as its compile.sh does (CONFIG2), links against
+    libnq_hip.so alone (cuda_n_queens, print_with_time, time_diff_ms) and, with the device
+    hidden, prints the reference's lines: the task count of (13, 5) and result -1."""
+    exe = tmp_path / "ref_main"
+    libdir = os.path.dirname(nq.LIB_PATH)
+    subprocess.check_call(["g++", "-O2", "-D_USE_CONFIG2_", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(REF_SRC, "main.cpp"), "-o", str(exe), "-L", libdir, "-lnq_hip",
+                           f"-Wl,-rpath,{libdir}"])
+    p = subprocess.run([str(exe), "13", "5"], capture_output=True, text=True, env=NO_GPU, timeout=60)
+    assert p.returncode == 0, p.stderr
+    stamp = r"\[\d{4}-\d\d-\d\d \d\d:\d\d:\d\d\.\d{3}\] "
+    assert re.search(stamp + r"Use [\d.]+ms to generate 15550 subproblems!", p.stdout), p.stdout
+    assert "Failed to find any gpu!" in p.stdout
+    assert re.search(stamp + r"cuda 13 queens result -1, calc time: \[[\d.]+ms\]", p.stdout), p.stdout
+    assert len(oracle.gen_tasks(13, 5)) == 15550
 
 
 def run_cli(*args, env=NO_GPU):
@@ -220,7 +279,7 @@ def test_default_and_solve_configs_leave_block_automatic():
     for N, rows, split in ((21, 6, 2), (22, 6, 3), (20, 6, 2), (19, 6, 2), (18, 6, 1), (17, 5, 2), (16, 4, 2),
                            (13, 5, 0), (28, 7, 3), (24, 10, 1)):
         s = nq.solve_config(N, rows)
-        assert s.split == split and s.block == 0 and s.stack_depth == N - 1 - min(N, rows + split), (N, rows)
+        assert s.split == split and s.block == 0 and s.stack_depth == N - 1 - rows, (N, rows)
 
 
 def test_config_resolve_needs_a_device():

@@ -149,3 +149,52 @@ def test_encode_prefix_vs_attack_check():
                 free = [c for c in range(N) if c not in cols]
                 n = sum(1 for rest in itertools.permutations(free) if not attacks([*cols, *rest]))
                 assert oracle.count_subtree(N, *enc) == n
+
+
+def _random_prefixes(rng, n, N_range, k_max_gap=0):
+    """Seeded non-attacking prefixes: N in N_range, k in 0..N (every depth, full boards too)."""
+    out = []
+    while len(out) < n:
+        N = int(rng.integers(*N_range))
+        k = int(rng.integers(0, N + 1 - k_max_gap))
+        cols = []
+        for row in range(k):
+            free = [c for c in range(N) if not attacks([*cols, c])]
+            if not free:
+                break
+            cols.append(int(free[rng.integers(len(free))]))
+        if len(cols) == k:
+            out.append((N, cols, oracle.encode_prefix(N, cols)))
+    return out
+
+
+def test_count_iterative_against_the_recursive_count_and_its_edge_cases():
+    """Below N-1 queens the iterative variant counts exactly the completions (brute force for
+    N <= 8, the recursion otherwise); a prefix of N-1 queens or a full board adds nothing,
+    which is what the reference's loop does (src/n_queens.cpp:29-31,56-66)."""
+    rng = np.random.default_rng(0x17E5)
+    seen_edge = 0
+    for N, cols, enc in _random_prefixes(rng, 400, (4, 13)):
+        got = oracle.count_iterative(N, *enc)
+        if len(cols) >= N - 1:
+            assert got == 0, (N, cols)
+            seen_edge += 1
+        else:
+            assert got == oracle.count_subtree(N, *enc), (N, cols)
+            if N <= 8:
+                free = [c for c in range(N) if c not in cols]
+                assert got == sum(1 for rest in itertools.permutations(free) if not attacks([*cols, *rest]))
+    assert seen_edge > 0
+    # The whole board: A000170 (n_queens_iterative is the reference's "a little slower" counter).
+    for N, want in ((8, 92), (10, 724), (12, 14200)):
+        assert oracle.count_iterative(N, 0, 0, 0) == want
+
+
+@pytest.mark.skipif(oracle.ref() is None, reason="oracle/_ref not built (no reference checkout)")
+def test_count_iterative_vs_reference_build():
+    """Against the reference's own n_queens_iterative, within its 16-frame stack (prefixes at
+    most 17 rows above the last row; N <= 14 here)."""
+    R = oracle.ref()
+    rng = np.random.default_rng(0x1E7A)
+    for N, cols, enc in _random_prefixes(rng, 300, (2, 15)):
+        assert oracle.count_iterative(N, *enc) == R.ref_count_iterative(N, *enc), (N, cols)
