@@ -286,7 +286,7 @@ def main() -> int:
     split = max(0, cfg.split)
     launch = nq.resolve_config(N, min(N, rows + split), 1 << 40, nq.Config(
         block=cfg.block, stack_depth=max(1, cfg.stack_depth - split), chunk=cfg.chunk, push=cfg.push,
-        waves_per_cu=cfg.waves_per_cu))
+        waves_per_cu=cfg.waves_per_cu, split=split))
 
     out = {
         "metric": METRIC, "value": nodes_per_s, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,

@@ -41,11 +41,14 @@ typedef struct nq_config {
                       /* the same push rule (one wave of each workgroup per SIMD), else 128 */
     int stack_depth;  /* LDS frames per lane; must be >= N - 1 - (prefix depth)           */
     int waves_per_cu; /* resident-wave target that sizes the persistent grid (0 = max)     */
-    int chunk;        /* tasks a wave claims per cursor grab (1..4096)                    */
+    int chunk;        /* tasks a wave claims per cursor grab (1..4096); with a split a     */
+                      /* wave claims whole tasks, one at a time by default                  */
     int grid;         /* explicit workgroup count (0 = from waves_per_cu)                  */
-    int split;        /* rows each task is expanded by on the device before the DFS (totals */
-                      /* only; ignored when per-task counts are requested). nq_solve: -1 = */
-                      /* automatic.  Every child keeps the exact count, order is free.      */
+    int split;        /* rows the DFS kernel walks below each task before its LDS DFS     */
+                      /* (0..4; -1 = automatic): a wave claims a whole task and hands the   */
+                      /* nodes `split` rows below it to its lanes as DFS roots, in registers */
+                      /* (no expanded list in memory).  Plain totals only: ignored when     */
+                      /* per-task or node counts are requested.                             */
     int push;         /* DFS push form: NQ_PUSH_AUTO (0) takes NQ_PUSH_ALWAYS when its spare */
                       /* stack level costs no resident waves, else NQ_PUSH_MASKED.  Both    */
                       /* give identical counts.                                              */
@@ -77,17 +80,18 @@ int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t 
 
 /* Fill cfg with defaults for (N, shallowest prefix depth d0): automatic workgroup size (block
  * 0), the smallest legal stack depth N-1-d0 (LDS per lane decides occupancy, so no frame is
- * wasted), chunk 16, push NQ_PUSH_AUTO, and a persistent grid of every workgroup that fits on
- * the device.  The reference's
+ * wasted), chunk 16, no split, push NQ_PUSH_AUTO, and a persistent grid of every workgroup that
+ * fits on the device.  The reference's
  * compile-time presets only decide which (N, rows) pairs its CLI accepts; the CLI here keeps
  * that check (src/main.cpp:16-19). */
 int nq_config_default(int N, int d0, nq_config *cfg);
 
 /* The configuration nq_solve(N, rows, NULL, ...) uses, in the form nq_solve and nq_count take
  * it back: stack_depth = N - 1 - rows bounds the depth-`rows` tasks (the count takes the split
- * rows off it), and split = the automatic device split of clamp(N - rows - R, 0, 3) rows, so
- * that the DFS starts R rows above the last row: R = (N+4)/2 up to N = 20, R = 13 from N = 21
- * on (12 LDS frames).  nq_solve(N, rows, &cfg, ...) with this cfg is nq_solve(N, rows, NULL). */
+ * rows off it), split = the automatic device split of clamp(N - rows - R, 0, 3) rows, so that
+ * the DFS starts R rows above the last row (R = (N+4)/2 up to N = 20, R = 13 from N = 21 on:
+ * 12 LDS frames), and chunk 1 with a split.  nq_solve(N, rows, &cfg, ...) with this cfg is
+ * nq_solve(N, rows, NULL). */
 int nq_solve_config(int N, int rows, nq_config *cfg);
 
 /* The launch configuration nq_count takes on the current device for ntasks tasks whose
@@ -112,21 +116,22 @@ int nq_generate(int N, int rows, uint32_t *tasks, uint64_t cap, uint64_t *ntasks
  * Counts all completions of every task (3 words each, all at the same N; prefixes of any
  * depth <= N).  *sum = sum of per-task counts (no symmetry fold).  per_task (nullable) gets the
  * per-task uint64 counts in task order — the reference's partial_sum vector.  nodes (nullable)
- * selects the node-counting kernel and returns the DFS node count.  stats is nullable. */
+ * selects the node-counting kernel and returns the DFS node count.  stats is nullable.  cfg
+ * NULL or split -1: the automatic split for the shallowest task. */
 int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg, uint64_t *sum,
              uint64_t *per_task, uint64_t *nodes, nq_stats *stats);
 
 /* Asynchronous device-pointer form of nq_count, for callers that keep the task list resident
  * in HBM: d_tasks (3*ntasks uint32), d_result (3 uint64: [0] += sum, [1] += nodes, [2] |= error
- * flags, bit 0 = a task needed more stack frames than configured, bit 1 = the device split
- * overflowed its buffer; the caller zeroes it) and d_per_task (nullable, ntasks uint64) are
- * device pointers on the current device; stream is a hipStream_t (NULL = default stream).
- * cfg->stack_depth, when set, bounds N-1-depth over the tasks as given (before any split).  Enqueues the work and returns;
- * the results are valid after the stream synchronises.  With cfg->split > 0 the call first
- * synchronizes the stream once per split row, to size that row's expansion buffer exactly.  Calls on one device share the
- * library's task cursor and are serialised by the library's device lock; each call must be
- * allowed to finish on its stream before a call on another stream of the same device is
- * issued. */
+ * flags, bit 0 = a task needed more stack frames than configured; the caller zeroes it) and
+ * d_per_task (nullable, ntasks uint64) are device pointers on the current device; stream is a
+ * hipStream_t (NULL = default stream).  cfg->stack_depth, when set, bounds N-1-depth over the
+ * tasks as given (before any split); cfg NULL or split -1 takes the automatic split for tasks of
+ * depth N-1-stack_depth (depth 1 without a stack depth).  Enqueues kernels only (no host
+ * synchronisation, no memset) and returns, so the call can be captured in a hipGraph; the
+ * results are valid after the stream synchronises.  Calls on one device share the library's task
+ * cursor and are serialised by the library's device lock; each call must be allowed to finish on
+ * its stream before a call on another stream of the same device is issued. */
 int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_config *cfg,
                     uint64_t *d_result, uint64_t *d_per_task, int count_nodes, void *stream);
 

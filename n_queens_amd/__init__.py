@@ -105,6 +105,10 @@ class Config:
         return _Config(self.block, self.stack_depth, self.waves_per_cu, self.chunk, self.grid, self.split,
                        self.push)
 
+    @staticmethod
+    def _from(c: _Config) -> "Config":
+        return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push)
+
 
 _lib = None
 
@@ -177,14 +181,14 @@ def device_info(device: int = 0) -> dict:
 def config_default(N: int, d0: int) -> Config:
     c = _Config()
     _check(lib().nq_config_default(N, d0, ctypes.byref(c)))
-    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push)
+    return Config._from(c)
 
 
 def solve_config(N: int, rows: int) -> Config:
     """The configuration nq_solve uses for (N, rows), including its device split."""
     c = _Config()
     _check(lib().nq_solve_config(N, rows, ctypes.byref(c)))
-    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push)
+    return Config._from(c)
 
 
 def resolve_config(N: int, d0: int, ntasks: int, cfg: Config | None = None) -> Config:
@@ -193,7 +197,7 @@ def resolve_config(N: int, d0: int, ntasks: int, cfg: Config | None = None) -> C
     c = _Config()
     cc = cfg._c() if cfg else None
     _check(lib().nq_config_resolve(N, d0, ntasks, ctypes.byref(cc) if cc else None, ctypes.byref(c)))
-    return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push)
+    return Config._from(c)
 
 
 def generate(N: int, rows: int) -> np.ndarray:

@@ -62,6 +62,20 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+PROBES = os.path.join(ROOT, "tests", "probes")
+
+
+def build_probes() -> None:
+    """The stand-alone GPU probes under tests/probes (timing programs, not part of the library)."""
+    for src in sorted(os.listdir(PROBES)):
+        if src.endswith(".hip"):
+            exe = os.path.join(PROBES, os.path.splitext(src)[0])
+            _run([HIPCC, f"--offload-arch={ARCH}", *DEVICE_FLAGS, "-O3", "-std=c++17", "-o", exe + ".tmp",
+                  os.path.join(PROBES, src)])
+            os.replace(exe + ".tmp", exe)
+
+
 if __name__ == "__main__":
     build()
+    build_probes()
     sys.exit(0)

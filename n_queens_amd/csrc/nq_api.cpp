@@ -113,12 +113,16 @@ static int check_cfg_shape(const nq_config &c) {
     return NQ_OK;
 }
 
+static int auto_split(int N, int rows);
+
 // Frames a prefix at depth d0 needs: one per depth d0..N-2 that can hold remaining candidates.
 static int frames_needed(int N, int d0) { return d0 + 1 >= N ? 0 : N - 1 - d0; }
 
-// Resolve defaults and the persistent grid for (N, shallowest prefix depth d0, ntasks).
-static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_config *in, bool nodes,
-                          bool per_task, nq_config *out) {
+// Resolve defaults and the persistent grid for (N, shallowest DFS root depth d0, ntasks).  With
+// split > 0 (plain totals) the kernel is split_kernel and d0 is the depth of the roots it walks
+// to, split rows below the tasks.
+static int resolve_config(int cus, int N, int d0, uint64_t ntasks, const nq_config *in, bool nodes,
+                          bool per_task, int split, nq_config *out) {
     nq_config c;
     nq_config_default(N, d0, &c);
     if (in) {
@@ -130,6 +134,9 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
         c.split = in->split;
         c.push = in->push;
     }
+    // With the split a wave claims whole tasks: one at a time unless told otherwise, so that no
+    // wave holds claimed tasks back while others run dry at the end.
+    if (split > 0 && !nodes && !per_task && !(in && in->chunk)) c.chunk = 1;
     int rc = check_cfg_shape(c);
     if (rc) return rc;
     if (c.stack_depth < frames_needed(N, d0)) {
@@ -143,8 +150,8 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
     // none of them.
     auto fit = [&](int block, int *push, int *bpc) -> int {
         int occ[2] = {0, 0};
-        NQ_HIP(dfs_occupancy(block, c.stack_depth, nodes, per_task, false, &occ[0]));
-        NQ_HIP(dfs_occupancy(block, c.stack_depth, nodes, per_task, true, &occ[1]));
+        NQ_HIP(dfs_occupancy(block, c.stack_depth, nodes, per_task, false, split, &occ[0]));
+        NQ_HIP(dfs_occupancy(block, c.stack_depth, nodes, per_task, true, split, &occ[1]));
         if (c.waves_per_cu > 0)
             for (int &o : occ) o = std::min(o, std::max(1, c.waves_per_cu / (block / 64)));
         *push = c.push ? c.push : (prefer_always_push(occ[0], occ[1]) ? NQ_PUSH_ALWAYS : NQ_PUSH_MASKED);
@@ -172,11 +179,46 @@ static int resolve_config(Device *d, int N, int d0, uint64_t ntasks, const nq_co
     }
     c.push = push;
     if (bpc <= 0) return fail(NQ_ERR_ARG, "configuration does not fit on a CU (LDS)");
-    if (c.grid == 0) c.grid = d->cus * bpc;
-    const uint64_t lanes_needed = (ntasks + c.block - 1) / c.block;
-    if (static_cast<uint64_t>(c.grid) > lanes_needed) c.grid = static_cast<int>(std::max<uint64_t>(1, lanes_needed));
+    if (c.grid == 0) c.grid = cus * bpc;
+    // Workgroups the tasks can keep busy: one lane per task, or with the split one wave per task
+    // (a wave walks a claimed task's roots on its 64 lanes).
+    const uint64_t per_wg = split > 0 && !nodes && !per_task ? static_cast<uint64_t>(c.block / 64) : c.block;
+    const uint64_t wgs_needed = (ntasks + per_wg - 1) / per_wg;
+    if (static_cast<uint64_t>(c.grid) > wgs_needed) c.grid = static_cast<int>(std::max<uint64_t>(1, wgs_needed));
+    c.split = split;
     *out = c;
     return NQ_OK;
+}
+
+// Rows the DFS kernel walks in registers below tasks of depth d0: cfg->split (a negative value or
+// no cfg: the automatic rule), never more than leaves the DFS two rows, at most MAX_SPLIT.
+static int effective_split(int N, int d0, const nq_config *cfg, int *split) {
+    int s = (cfg && cfg->split >= 0) ? cfg->split : auto_split(N, d0);
+    if (s > MAX_SPLIT) return fail(NQ_ERR_ARG, "split must be -1 (automatic) or 0..4");
+    *split = std::max(0, std::min(s, N - 2 - d0));
+    return NQ_OK;
+}
+
+// The launch nq_count makes for a task list (shallowest depth d0) on a device with `cus` compute
+// units: the split (cfg NULL or split -1: automatic) and the resolved configuration, for callers
+// that bring their own device memory (nq_static.hip).  Needs no allocation.
+int plan_count(int cus, int N, int d0, uint64_t ntasks, const nq_config *cfg, bool nodes, bool per_task,
+               nq_config *out) {
+    nq_config in = cfg ? *cfg : nq_config{};
+    if (!cfg) in.split = -1;
+    int split = 0;
+    if (!per_task && !nodes) {
+        int rc = effective_split(N, d0, &in, &split);
+        if (rc) return rc;
+    }
+    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - split);
+    return resolve_config(cus, N, std::min(N, d0 + split), ntasks, &in, nodes, per_task, split, out);
+}
+
+int task_min_depth(int N, const uint32_t *tasks, uint64_t ntasks) {
+    int d0 = N;
+    for (uint64_t i = 0; i < ntasks; ++i) d0 = std::min(d0, __builtin_popcount(tasks[3 * i]));
+    return d0;
 }
 
 static int ensure_tasks(Device *d, uint64_t n) {
@@ -290,11 +332,6 @@ static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, const 
     return NQ_OK;
 }
 
-static int min_depth(int N, const uint32_t *tasks, uint64_t ntasks) {
-    int d0 = N;
-    for (uint64_t i = 0; i < ntasks; ++i) d0 = std::min(d0, __builtin_popcount(tasks[3 * i]));
-    return d0;
-}
 
 // Rows nq_solve expands on the device (0..3): the DFS then starts dfs_rows(N) rows above the
 // last row, i.e. with dfs_rows(N) - 1 LDS frames.  Measured on one MI355X (split + DFS, final
@@ -315,9 +352,9 @@ static int auto_split(int N, int rows) {
 // depth `rows` (src/n_queens.cpp:74-113) are every legal continuation of its tasks at depth
 // min(rows, 2): only row 0 (low half) and, for odd N, rows 0-1 (middle column, then the low
 // side) are restricted.  The host generates those few seeds; the device expands them to depth
-// `rows` (exact sizes, order free: the same multiset of tasks), then by the DFS split, and
-// counts.  on_generated (nullable) is called once the depth-`rows` list exists, with its task
-// count and the generation time.
+// `rows` (exact sizes, order free: the same multiset of tasks) and counts them, the DFS kernel
+// walking the split rows below each task in registers.  on_generated (nullable) is called once
+// the depth-`rows` list exists, with its task count and the generation time.
 int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats,
                     void (*on_generated)(void *ctx, uint64_t ntasks, double ms), void *ctx) {
     const double t0 = now_ms();
@@ -332,7 +369,7 @@ int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_s
     generate_tasks(N, seed_rows, seeds);
     const uint64_t nseeds = seeds.size() / 3;
     nq_config in = cfg ? *cfg : nq_config{};
-    const int split = (!cfg || in.split < 0) ? auto_split(N, rows) : in.split;
+    if (!cfg) in.split = -1;
 
     std::lock_guard<std::mutex> g(d->mu);
     rc = ensure_tasks(d, nseeds);
@@ -362,20 +399,17 @@ int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_s
     const uint64_t ntasks_rows = n;  // the reference's task count for (N, rows)
     if (on_generated) on_generated(ctx, ntasks_rows, t3 - t0 - (t2 - t1));
 
-    // Split for the DFS, then count.
-    NQ_HIP(hipEventRecord(d->ev0, d->stream));
-    int lev = 0;
-    if (split && n) {
-        rc = enqueue_split(d, N, seed_rows + gen, t, n_dev, n, split, gen & 1, false, d->stream, &t, &n_dev, &n,
-                           &lev);
-        if (rc) return rc;
-    }
-    // cfg->stack_depth bounds the depth-`rows` tasks, as for nq_count; the split takes lev rows off.
-    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - lev);
-    in.split = 0;
-    nq_config c;
-    rc = resolve_config(d, N, std::min(N, seed_rows + gen + lev), n, &in, false, false, &c);
+    // Count, the kernel walking `split` rows below every task in registers.  cfg->stack_depth
+    // bounds the depth-`rows` tasks, as for nq_count; the DFS roots sit `split` rows lower.
+    const int d_gen = std::min(N, seed_rows + gen);
+    int split = 0;
+    rc = effective_split(N, d_gen, &in, &split);
     if (rc) return rc;
+    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - split);
+    nq_config c;
+    rc = resolve_config(d->cus, N, std::min(N, d_gen + split), n, &in, false, false, split, &c);
+    if (rc) return rc;
+    NQ_HIP(hipEventRecord(d->ev0, d->stream));
     const size_t nwaves = static_cast<size_t>(c.grid) * (c.block / 64);
     if (stats) {
         rc = ensure_trace(d, 2 * nwaves);
@@ -384,8 +418,9 @@ int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_s
     LaunchArgs a{t, n, N, c.block, c.grid, c.stack_depth, c.chunk, d->state, nullptr, d->result, false,
                  stats ? d->trace : nullptr};
     a.ntasks_dev = n_dev;
-    a.xflags = (gen + lev) ? d->xcount + 2 : nullptr;
+    a.xflags = gen ? d->xcount + 2 : nullptr;
     a.apush = c.push == NQ_PUSH_ALWAYS;
+    a.split = split;
     e = launch_count(a, d->stream);
     if (e != hipSuccess) return fail_hip(e, "launch_count");
     NQ_HIP(hipEventRecord(d->ev1, d->stream));
@@ -490,29 +525,24 @@ int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_co
     // prefix the caller allows (cfg->stack_depth), else for depth 1; the kernel flags and skips
     // any task that would need more.
     const int d0 = (cfg && cfg->stack_depth) ? std::max(0, N - 1 - cfg->stack_depth) : 1;
-    const int split = (cfg && !d_per_task && !count_nodes) ? std::max(0, cfg->split) : 0;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    std::lock_guard<std::mutex> g(d->mu);
-    const uint32_t *t = d_tasks;
-    const unsigned long long *n_dev = nullptr;
-    uint64_t bound = ntasks;
-    int levels = 0;
-    if (split) {
-        hipError_t e = launch_zero(d->xcount + 2, 1, s);
-        if (e != hipSuccess) return fail_hip(e, "launch_zero");
-        rc = enqueue_split(d, N, d0, d_tasks, nullptr, ntasks, split, 0, false, s, &t, &n_dev, &bound, &levels);
+    nq_config in = cfg ? *cfg : nq_config{};
+    if (!cfg) in.split = -1;
+    int split = 0;
+    if (!d_per_task && !count_nodes) {
+        rc = effective_split(N, d0, &in, &split);
         if (rc) return rc;
     }
-    nq_config in = cfg ? *cfg : nq_config{};
-    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - levels);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> g(d->mu);
+    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - split);
     nq_config c;
-    rc = resolve_config(d, N, d0 + levels, bound, &in, count_nodes != 0, d_per_task != nullptr, &c);
+    rc = resolve_config(d->cus, N, std::min(N, d0 + split), ntasks, &in, count_nodes != 0, d_per_task != nullptr,
+                        split, &c);
     if (rc) return rc;
-    LaunchArgs a{t, bound, N, c.block, c.grid, c.stack_depth, c.chunk, d->state, d_per_task, d_result,
+    LaunchArgs a{d_tasks, ntasks, N, c.block, c.grid, c.stack_depth, c.chunk, d->state, d_per_task, d_result,
                  count_nodes != 0};
-    a.ntasks_dev = n_dev;
-    a.xflags = levels ? d->xcount + 2 : nullptr;
     a.apush = c.push == NQ_PUSH_ALWAYS;
+    a.split = split;
     hipError_t e = launch_count(a, s);
     if (e != hipSuccess) return fail_hip(e, "launch_count");
     return NQ_OK;
@@ -527,8 +557,14 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     Device *d = nullptr;
     int rc = current_device(&d);
     if (rc) return rc;
-    const int d0 = min_depth(N, tasks, ntasks);
-    const int split = (cfg && !per_task && !nodes) ? std::max(0, cfg->split) : 0;
+    const int d0 = task_min_depth(N, tasks, ntasks);
+    nq_config in = cfg ? *cfg : nq_config{};
+    if (!cfg) in.split = -1;
+    int split = 0;
+    if (!per_task && !nodes) {
+        rc = effective_split(N, d0, &in, &split);
+        if (rc) return rc;
+    }
 
     std::lock_guard<std::mutex> g(d->mu);
     rc = ensure_tasks(d, ntasks);
@@ -543,33 +579,21 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     NQ_HIP(hipMemsetAsync(d->result, 0, 3 * sizeof(uint64_t), d->stream));
     NQ_HIP(hipStreamSynchronize(d->stream));
     const double t2 = now_ms();
-    NQ_HIP(hipEventRecord(d->ev0, d->stream));
-    const uint32_t *t = d->tasks;
-    const unsigned long long *n_dev = nullptr;
-    uint64_t bound = ntasks;
-    int levels = 0;
-    if (split) {
-        hipError_t e = launch_zero(d->xcount + 2, 1, d->stream);
-        if (e != hipSuccess) return fail_hip(e, "launch_zero");
-        rc = enqueue_split(d, N, d0, d->tasks, nullptr, ntasks, split, 0, false, d->stream, &t, &n_dev, &bound,
-                           &levels);
-        if (rc) return rc;
-    }
-    nq_config in = cfg ? *cfg : nq_config{};
-    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - levels);
+    if (in.stack_depth) in.stack_depth = std::max(1, in.stack_depth - split);
     nq_config c;
-    rc = resolve_config(d, N, std::min(N, d0 + levels), bound, &in, nodes != nullptr, per_task != nullptr, &c);
+    rc = resolve_config(d->cus, N, std::min(N, d0 + split), ntasks, &in, nodes != nullptr, per_task != nullptr,
+                        split, &c);
     if (rc) return rc;
+    NQ_HIP(hipEventRecord(d->ev0, d->stream));
     const size_t nwaves = static_cast<size_t>(c.grid) * (c.block / 64);
     if (stats) {
         rc = ensure_trace(d, 2 * nwaves);
         if (rc) return rc;
     }
-    LaunchArgs a{t, bound, N, c.block, c.grid, c.stack_depth, c.chunk, d->state,
+    LaunchArgs a{d->tasks, ntasks, N, c.block, c.grid, c.stack_depth, c.chunk, d->state,
                  per_task ? d->per_task : nullptr, d->result, nodes != nullptr, stats ? d->trace : nullptr};
-    a.ntasks_dev = n_dev;
-    a.xflags = levels ? d->xcount + 2 : nullptr;
     a.apush = c.push == NQ_PUSH_ALWAYS;
+    a.split = split;
     hipError_t e = launch_count(a, d->stream);
     if (e != hipSuccess) return fail_hip(e, "launch_count");
     NQ_HIP(hipEventRecord(d->ev1, d->stream));
@@ -585,7 +609,6 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
         NQ_HIP(hipMemcpyAsync(tr.data(), d->trace, tr.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, d->stream));
     }
     NQ_HIP(hipStreamSynchronize(d->stream));
-    if (res[2] & 2) return fail(NQ_ERR_STATE, "device expansion overflowed its buffer");
     if ((st.error | res[2]) & DEV_ERR_LDS) return fail(NQ_ERR_STATE, "DFS kernel LDS does not start at address 0");
     if (st.error || (res[2] & 1)) return fail(NQ_ERR_DEPTH, "a task needed more stack frames than configured");
     *sum = res[0];
@@ -612,6 +635,7 @@ int nq_solve_config(int N, int rows, nq_config *cfg) {
     // split rows off it for the DFS after the split).
     nq_config_default(N, rows, cfg);
     cfg->split = auto_split(N, rows);
+    if (cfg->split > 0) cfg->chunk = 1;  // whole tasks per wave claim
     return NQ_OK;
 }
 
@@ -622,7 +646,8 @@ int nq_config_resolve(int N, int d0, uint64_t ntasks, const nq_config *cfg, nq_c
     int rc = current_device(&d);
     if (rc) return rc;
     std::lock_guard<std::mutex> g(d->mu);
-    return resolve_config(d, N, d0, ntasks, cfg, false, false, out);
+    const int split = cfg && cfg->split > 0 ? std::min(cfg->split, MAX_SPLIT) : 0;
+    return resolve_config(d->cus, N, d0, ntasks, cfg, false, false, split, out);
 }
 
 int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats) {

@@ -36,6 +36,7 @@
 #include <algorithm>
 
 #include "nq_kernels.h"
+#include "nq_walk.h"
 
 namespace nq {
 
@@ -383,6 +384,126 @@ dfs_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint
     }
 }
 
+// The first active lane of the wave (all lanes are active wherever this is called).
+__device__ __forceinline__ uint32_t wave_leader() {
+    return static_cast<uint32_t>(__builtin_ctzll(__ballot(1)));
+}
+
+// Plain totals with the device split done in registers: each wave claims whole tasks from the
+// cursor (`chunk` at a time) and walks the top S rows of a task's subtree (TaskWalk, the same on
+// every lane: scalar registers); every node S rows below the task becomes one lane's DFS root
+// when that lane's previous root is exhausted.  The expanded list never exists in memory, the
+// launch needs no host synchronisation, and each task is read once.  A task's subtree stays on
+// the wave that claimed it (a few hundred roots at most, spread over its 64 lanes).  The DFS is
+// dfs_kernel's hot loop with the per-wave board counter.
+template <bool APUSH, int S>
+__global__ void __launch_bounds__(MAX_BLOCK)
+split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint32_t chunk, uint32_t stack_depth,
+             DevState *__restrict__ st, uint64_t *__restrict__ trace, const unsigned long long *__restrict__ ntasks_dev) {
+    if (ntasks_dev) ntasks = min(ntasks, static_cast<uint64_t>(*ntasks_dev));
+    extern __shared__ uint4 lds[];  // lds_layout(): dead frame, stack, wq (unused), red
+    if (!lds_base_ok(lds, st)) return;
+    const LdsLayout L = lds_layout(lds, stack_depth + (APUSH ? 1u : 0u));
+    auto *red = L.red;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t t_start = trace ? wall_clock64() : 0;
+    put_dead_frame(lds);
+    const uint32_t last = (1u << N) - 1u;
+    const uint32_t stride = blockDim.x * static_cast<uint32_t>(sizeof(uint4));
+    const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(L.stack + threadIdx.x));
+    uint32_t sp = base;
+    uint32_t c = 0, l = 0, r = 0, v = 0;  // the frame being expanded
+    uint32_t cnt = 0, nd = 0;             // per-lane counters of hot_loop (unused: per-wave count)
+    uint32_t wcnt = 0;                    // completed boards of the wave since the last fold
+    uint64_t wsum = 0;                    // the wave's count: folded wcnt + last-row and edge counts
+    bool has = true;                      // lane still has (or may get) a DFS root
+
+    // Wave-uniform state: the walk over the current task and the wave's claimed task range.
+    nq::TaskWalk<S> walk;
+    uint64_t next = 0, end = 0;
+    bool drained = false;  // no task left to claim
+
+    for (;;) {
+        // Lanes parked on the dead frame take the next roots, one each, lowest lane first.
+        uint64_t need = __ballot(has && v == 0);
+        while (need) {
+            uint32_t dc = 0, dl = 0, dr = 0, dv = 0;
+            bool got = false;
+            while (!drained) {
+                uint64_t direct = 0;
+                got = walk.next(N, last, &dc, &dl, &dr, &dv, &direct);
+                wsum += direct;
+                if (got) break;
+                if (next >= end) {
+                    const uint32_t leader = wave_leader();
+                    uint64_t g = 0;
+                    if (lane == leader) g = atomicAdd(&st->cursor, static_cast<unsigned long long>(chunk));
+                    g = first_lane_u64(g);
+                    next = g;
+                    end = g + chunk;
+                }
+                if (next >= ntasks) {
+                    drained = true;
+                    break;
+                }
+                const uint64_t t = next++;
+                const uint32_t c0 = __builtin_amdgcn_readfirstlane(tasks[3 * t + 0]);
+                const uint32_t l0 = __builtin_amdgcn_readfirstlane(tasks[3 * t + 1]);
+                const uint32_t r0 = __builtin_amdgcn_readfirstlane(tasks[3 * t + 2]);
+                const uint32_t v0 = last & ~(c0 | l0 | r0);
+                const uint32_t d0 = __popc(c0);
+                if (d0 + 1u >= N || v0 == 0) {
+                    // Prefix at the last row (or full), or no legal next queen: counted here
+                    // (src/n_queens.cpp:10-13 counts a full board).
+                    wsum += d0 >= N ? (c0 == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
+                    continue;
+                }
+                if (d0 + S + 1u < N && N - 1u - (d0 + S) > stack_depth) {
+                    // Its roots would need more frames than configured: flag and skip.
+                    if (lane == wave_leader()) atomicOr(&st->error, DEV_ERR_DEPTH);
+                    continue;
+                }
+                walk.start(c0, l0, r0, v0, d0);
+            }
+            if (!got) {
+                if ((need >> lane) & 1ull) has = false;
+                break;
+            }
+            const uint32_t sel = static_cast<uint32_t>(__builtin_ctzll(need));
+            need &= need - 1;
+            if (lane == sel) {
+                c = dc | ~last;
+                l = dl;
+                r = dr;
+                v = dv;
+                sp = base;
+            }
+        }
+        const uint64_t live = __ballot(has);
+        if (live == 0) break;
+        hot_loop<false, APUSH, true>(c, l, r, v, sp, cnt, wcnt, nd, live, stride);
+        wsum += wcnt;
+        wcnt = 0;
+    }
+
+    if (trace && lane == 0) {
+        const uint64_t w = static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + wave;
+        trace[2 * w] = t_start;
+        trace[2 * w + 1] = wall_clock64();
+    }
+    if (lane == 0) {
+        red[wave][0] = wsum;
+        red[wave][1] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) s += red[w][0];
+        atomicAdd(&st->total, s);
+    }
+}
+
 // Batched instances: tasks are {cur, left, right, inst << 6 | twice << 5 | N}; N varies per
 // lane (the iteration does not depend on it), and every finished task's count (doubled when
 // `twice` is set: a mirror-symmetric half of an empty board) goes to counts[inst].  Tasks of one
@@ -615,6 +736,22 @@ const void *select_kernel(bool nodes, bool per_task, bool apush) {
     return k[nodes][per_task][apush];
 }
 
+// split_kernel for S = split rows (1..MAX_SPLIT).
+const void *select_split_kernel(int split, bool apush) {
+    const void *k[MAX_SPLIT][2] = {
+        {reinterpret_cast<const void *>(&split_kernel<false, 1>), reinterpret_cast<const void *>(&split_kernel<true, 1>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 2>), reinterpret_cast<const void *>(&split_kernel<true, 2>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 3>), reinterpret_cast<const void *>(&split_kernel<true, 3>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 4>), reinterpret_cast<const void *>(&split_kernel<true, 4>)}};
+    return k[split - 1][apush ? 1 : 0];
+}
+
+// The DFS kernel of a launch: split_kernel for plain totals with a device split, else dfs_kernel.
+const void *select_count_kernel(bool nodes, bool per_task, bool apush, int split) {
+    if (split > 0 && !nodes && !per_task) return select_split_kernel(split, apush);
+    return select_kernel(nodes, per_task, apush);
+}
+
 const void *select_batch_kernel(bool apush) {
     return apush ? reinterpret_cast<const void *>(&batch_kernel<true>)
                  : reinterpret_cast<const void *>(&batch_kernel<false>);
@@ -673,9 +810,9 @@ size_t dfs_lds_bytes(int block, int levels) {
            2 * MAX_WAVES_PER_BLOCK * 2 * sizeof(unsigned long long);
 }
 
-hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task, bool apush,
+hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task, bool apush, int split,
                          int *blocks_per_cu) {
-    const void *k = select_kernel(count_nodes, per_task, apush);
+    const void *k = select_count_kernel(count_nodes, per_task, apush, split);
     const size_t lds = dfs_lds_bytes(block, stack_depth + (apush ? 1 : 0));
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
@@ -715,7 +852,9 @@ hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t lds = dfs_lds_bytes(a.block, a.stack_depth + (a.apush ? 1 : 0));
-    const void *k = select_kernel(a.count_nodes, a.per_task != nullptr, a.apush);
+    if (a.split < 0 || a.split > MAX_SPLIT) return hipErrorInvalidValue;
+    const bool split = a.split > 0 && !a.count_nodes && !a.per_task;
+    const void *k = select_count_kernel(a.count_nodes, a.per_task != nullptr, a.apush, a.split);
     e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     uint32_t N = static_cast<uint32_t>(a.N);
@@ -728,7 +867,8 @@ hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
     uint64_t *trace = a.trace;
     const unsigned long long *ntasks_dev = a.ntasks_dev;
     void *args[] = {&tasks, &ntasks, &N, &chunk, &sd, &state, &per_task, &trace, &ntasks_dev};
-    e = hipLaunchKernel(k, dim3(a.grid), dim3(a.block), args, lds, stream);
+    void *split_args[] = {&tasks, &ntasks, &N, &chunk, &sd, &state, &trace, &ntasks_dev};
+    e = hipLaunchKernel(k, dim3(a.grid), dim3(a.block), split ? split_args : args, lds, stream);
     if (e != hipSuccess) return e;
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -23,6 +23,15 @@ void generate_from(int N, bool odd_part, uint32_t cur, uint32_t left, uint32_t r
 int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats,
                     void (*on_generated)(void *ctx, uint64_t ntasks, double ms), void *ctx);
 
+// nq_count's launch plan for a task list of shallowest depth d0 on a device with `cus` compute
+// units (split resolved from cfg, cfg NULL = automatic), without touching device memory
+// (nq_api.cpp).  out->split holds the split rows the DFS kernel walks.
+int plan_count(int cus, int N, int d0, uint64_t ntasks, const nq_config *cfg, bool nodes, bool per_task,
+               nq_config *out);
+
+// Shallowest prefix depth (queens placed) over a task list.
+int task_min_depth(int N, const uint32_t *tasks, uint64_t ntasks);
+
 // Thread-local error reporting for the C ABI.
 int fail(int status, const std::string &msg);
 int fail_hip(int hip_error, const char *what);

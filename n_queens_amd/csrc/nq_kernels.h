@@ -11,6 +11,8 @@ namespace nq {
 constexpr int MAX_N = 28;
 constexpr int MAX_BLOCK = 256;
 constexpr int MAX_WAVES_PER_BLOCK = MAX_BLOCK / 64;
+// Rows the DFS kernel can enumerate in registers below each task (split_kernel).
+constexpr int MAX_SPLIT = 4;
 
 // Device-resident per-launch state, owned by the library (one per device).
 struct DevState {
@@ -40,6 +42,9 @@ struct LaunchArgs {
     const unsigned long long *ntasks_dev = nullptr;  // nullable: task count on the device (<= ntasks)
     const unsigned long long *xflags = nullptr;      // nullable: expansion flags folded into result[2]
     bool apush = false;  // unconditional push (the launch allocates stack_depth + 1 levels)
+    // Rows enumerated in registers below every task before its DFS (0..MAX_SPLIT; plain totals
+    // only: ignored with count_nodes or per_task).  stack_depth then bounds the tasks' roots.
+    int split = 0;
 };
 
 struct BatchArgs {
@@ -89,8 +94,8 @@ hipError_t launch_child_count(const uint32_t *in, uint64_t n_max, const unsigned
 // Zero n 64-bit words with a kernel (graph-capturable, unlike a memset node).
 hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t stream);
 
-// Resident workgroups per CU for the DFS kernel at (block, stack_depth, push form).
-hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task, bool apush,
+// Resident workgroups per CU for the DFS kernel at (block, stack_depth, push form, split rows).
+hipError_t dfs_occupancy(int block, int stack_depth, bool count_nodes, bool per_task, bool apush, int split,
                          int *blocks_per_cu);
 
 // Dynamic LDS of a launch with `levels` stack levels.

@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+# torch bundles its own HIP runtime under the same soname as /opt/rocm's (libamdhip64.so.7), and
+# the first one loaded serves the whole process.  GPU tests that use torch tensors need torch's,
+# so torch is imported before anything loads libnq_hip.so (bench.py does the same).
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -127,7 +127,8 @@ def test_cli_end_to_end():
 
 @pytest.mark.parametrize("N,rows", [(10, 2), (13, 3), (16, 4), (17, 5)])
 def test_device_split_keeps_totals(N, rows, published):
-    """Tasks expanded on the device by 1..3 rows give the same totals."""
+    """Tasks split on the device by 1..3 rows (walked in registers by the DFS kernel) give the
+    same totals."""
     t = nq.generate(N, rows)
     base = nq.count(N, t).sum
     assert 2 * base == published["solutions"][N]
@@ -139,7 +140,7 @@ def test_device_split_keeps_totals(N, rows, published):
 
 
 def test_device_split_mixed_depth_list():
-    """Expansion of a list holding full boards, depth N-1 and dead prefixes keeps every count."""
+    """A split of a list holding full boards, depth N-1 and dead prefixes keeps every count."""
     N = 8
     rows = []
     for cols in ([0, 4, 7, 5, 2, 6, 1, 3], [0, 4, 7, 5, 2, 6, 1], [0, 2, 4], [1, 3, 5, 7], [2, 4, 1, 7, 0], [], [3]):
@@ -153,8 +154,8 @@ def test_device_split_mixed_depth_list():
 @pytest.mark.parametrize("chunk", [2, 8, 64, 1000])
 @pytest.mark.parametrize("split", [0, 1, 2])
 def test_chunked_claims_with_device_split(chunk, split, published):
-    """Chunked cursor claims stay exact when many claimed tasks die at once (after a device
-    split most expanded tasks are dead, so lanes retry the wave's LDS range many times)."""
+    """Chunked cursor claims stay exact, per lane (split 0: lanes retry the wave's LDS range
+    when claimed tasks are dead) and per wave (split > 0: whole tasks per wave)."""
     for N, rows, block in ((17, 5, 64), (15, 3, 256), (12, 2, 128)):
         t = nq.generate(N, rows)
         r = nq.count(N, t, cfg=nq.Config(block=block, chunk=chunk, split=split))

@@ -1,0 +1,92 @@
+"""GPU parity of the device split done in registers (split_kernel): each wave claims whole tasks
+and walks the top `split` rows of their subtrees, handing the roots to its lanes' DFS.  Counts
+are integers and must match bit for bit."""
+import numpy as np
+import pytest
+
+import n_queens_amd as nq
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N,rows", [(12, 2), (14, 3), (16, 4), (17, 5), (18, 6)])
+def test_split_kernel_totals(N, rows, published):
+    """Every split depth 1..4, both push forms, narrow and wide workgroups, single and chunked
+    task claims: the same total as A000170."""
+    t = nq.generate(N, rows)
+    for split in (1, 2, 3, 4):
+        for push, block, chunk in ((nq.NQ_PUSH_MASKED, 64, 1), (nq.NQ_PUSH_ALWAYS, 256, 5), (0, 0, 0)):
+            r = nq.count(N, t, cfg=nq.Config(split=split, push=push, block=block, chunk=chunk))
+            assert 2 * r.sum == published["solutions"][N], (N, rows, split, push, block, chunk)
+
+
+def test_split_kernel_mixed_depths_vs_oracle():
+    """A list with tasks of every depth (full boards, last-row and dead prefixes, the empty
+    board), the stack sized for the shallowest: the walk counts the deep ones itself."""
+    N = 12
+    rng = np.random.default_rng(12)
+    parts = [oracle.gen_tasks(N, d) for d in range(1, N + 1)]
+    t = np.concatenate([p[rng.choice(len(p), min(len(p), 50), replace=False)] for p in parts if len(p)])
+    t = np.concatenate([t, np.zeros((1, 3), np.uint32), np.array([[0b11, 0b1100, 0]], np.uint32)])
+    rng.shuffle(t)
+    want = oracle.count_tasks(N, t)
+    for split in range(0, 5):
+        assert nq.count(N, t, cfg=nq.Config(split=split)).sum == want, split
+
+
+def test_split_ignored_for_per_task_and_node_counts():
+    """Per-task and node counts need one lane per task: the split does not apply to them."""
+    N, rows = 14, 3
+    t = nq.generate(N, rows)
+    r = nq.count(N, t, cfg=nq.Config(split=3), per_task=True, nodes=True)
+    assert np.array_equal(r.per_task, oracle.count_tasks(N, t, per_task=True)[1])
+    assert r.nodes == oracle.count_tasks_nodes(N, t)
+
+
+def test_split_above_the_limit_is_an_argument_error():
+    with pytest.raises(nq.NQError) as e:
+        nq.count(12, nq.generate(12, 2), cfg=nq.Config(split=5))
+    assert e.value.status == nq.NQ_ERR_ARG
+
+
+@pytest.mark.parametrize("split", [2, 3, 4])
+def test_n20_split_vs_published(split, published):
+    total, st = nq.solve(20, 6, nq.Config(split=split))
+    assert st["ntasks"] == 2967560
+    assert total == published["solutions"][20], split
+
+
+def test_count_device_split_is_captured_in_a_graph(published):
+    """nq_count_device with a split enqueues kernels only (no host synchronisation, no memset),
+    so it can be captured in a graph: every replay, with eager work between replays, gives the
+    eager count."""
+    import torch
+    N, rows = 17, 5
+    tasks = nq.generate(N, rows)
+    dev = torch.device("cuda:0")
+    d_tasks = torch.from_numpy(tasks.view(np.int32).reshape(-1)).to(dev)
+    d_res = torch.zeros(3, dtype=torch.int64, device=dev)
+    cfg = nq.Config(split=2, stack_depth=N - 1 - rows)
+    s = torch.cuda.Stream(device=dev)
+    # Eager count (also initialises the library's device state outside the capture).
+    nq.count_device(N, d_tasks.data_ptr(), len(tasks), d_res.data_ptr(), cfg=cfg, stream=s.cuda_stream)
+    s.synchronize()
+    eager = d_res.cpu().numpy().view(np.uint64).copy()
+    assert 2 * int(eager[0]) == published["solutions"][N] and int(eager[2]) == 0
+    g = torch.cuda.CUDAGraph()
+    d_res.zero_()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+        nq.count_device(N, d_tasks.data_ptr(), len(tasks), d_res.data_ptr(), cfg=cfg,
+                        stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for i in range(4):
+        d_res.zero_()
+        junk = torch.arange(1 << 20, device=dev).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        got = d_res.cpu().numpy().view(np.uint64)
+        assert np.array_equal(got, eager), (i, got, eager)
+        assert int(junk) == (1 << 20) * ((1 << 20) - 1) // 2

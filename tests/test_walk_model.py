@@ -1,0 +1,68 @@
+"""CPU model of the device split in registers (split_kernel): the kernel's own TaskWalk
+(n_queens_amd/csrc/nq_walk.h) inside an emulation of its claim/refill control flow
+(tests/walk_model.cpp).  For every case the model must claim each task exactly once, hand out
+only roots that fit the configured stack, terminate, and reach the oracle's total."""
+import itertools
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def model(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("walk") / "walk_model"
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "n_queens_amd", "csrc"),
+                           os.path.join(ROOT, "tests", "walk_model.cpp"), "-o", str(exe)])
+    return str(exe)
+
+
+def run_model(model, tmp_path, N, tasks, split, chunk, waves, stack):
+    f = tmp_path / "tasks.bin"
+    np.ascontiguousarray(tasks, dtype=np.uint32).tofile(f)
+    out = subprocess.check_output([model, str(N), str(split), str(chunk), str(waves), str(stack), str(f)],
+                                  timeout=120).split()
+    total, flags, roots, claims_ok, frames_ok, done = (int(x) for x in out)
+    return total, flags, roots, claims_ok == 1, frames_ok == 1, done == 1
+
+
+@pytest.mark.parametrize("N", range(5, 14))
+def test_walk_model_totals(model, tmp_path, N):
+    for rows, split in itertools.product(range(1, N - 1), range(1, 5)):
+        tasks = oracle.gen_tasks(N, rows)
+        want = oracle.count_tasks(N, tasks)
+        stack = max(1, N - 1 - (rows + split))
+        for chunk, waves in ((1, 1), (3, 7), (16, 3)):
+            total, flags, roots, claims, frames, done = run_model(model, tmp_path, N, tasks, split, chunk, waves,
+                                                                  stack)
+            assert (claims, frames, done, flags) == (True, True, True, 0), (N, rows, split, chunk, waves)
+            assert total == want, (N, rows, split, chunk, waves, total, want)
+
+
+def test_walk_model_mixed_depths_and_edge_tasks(model, tmp_path):
+    """Tasks of every depth in one list (full boards, last-row prefixes, dead prefixes, shallow
+    ones), with the stack sized for the shallowest: the walk counts the deep ones directly."""
+    N = 11
+    rng = np.random.default_rng(5)
+    parts = [oracle.gen_tasks(N, d) for d in range(1, N + 1)]
+    tasks = np.concatenate([p[rng.choice(len(p), min(len(p), 40), replace=False)] for p in parts if len(p)])
+    tasks = np.concatenate([tasks, np.array([[0b11, 0b1100, 0], [0, 0, 0]], dtype=np.uint32)])  # dead, empty
+    rng.shuffle(tasks)
+    want = oracle.count_tasks(N, tasks)
+    for split in range(1, 5):
+        stack = N - 1 - split  # the empty board (depth 0) is the shallowest
+        total, flags, _, claims, frames, done = run_model(model, tmp_path, N, tasks, split, 2, 5, stack)
+        assert (claims, frames, done, flags, total) == (True, True, True, 0, want), split
+
+
+def test_walk_model_flags_a_stack_too_small(model, tmp_path):
+    N, rows, split = 12, 3, 2
+    tasks = oracle.gen_tasks(N, rows)
+    total, flags, roots, claims, frames, done = run_model(model, tmp_path, N, tasks, split, 4, 3,
+                                                          N - 1 - (rows + split) - 1)
+    assert flags == 1 and roots == 0 and claims and done

@@ -1,0 +1,203 @@
+// CPU model of split_kernel's control flow (n_queens_amd/csrc/nq_dfs.hip), built around the
+// kernel's own TaskWalk (n_queens_amd/csrc/nq_walk.h, compiled here for the host).
+//
+// Waves share one task cursor and claim `chunk` tasks at a time; each wave walks its task's top
+// `split` rows and hands every root to its lowest lane that is parked (no root, or the last one
+// exhausted), exactly as the kernel's refill loop does.  A lane's DFS is modelled by its work:
+// the root's expansions, consumed 16 per trip, and its count, the root's completions (both from
+// a plain recursion).  Waves advance round-robin, one refill + trips-until-a-lane-parks at a
+// time, so lanes finish at different times and waves interleave their claims.
+//
+// Checks, reported on stdout as one line "total flags roots claims_ok frames_ok terminated":
+//   * every task index is claimed exactly once;
+//   * every handed-out root needs at most stack_depth frames (N - 1 - its depth);
+//   * every wave terminates.
+// The total is compared with the oracle by tests/test_walk_model.py.
+//
+//   walk_model N split chunk waves stack_depth taskfile     (taskfile: uint32 triples)
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <vector>
+
+#include "nq_walk.h"
+
+namespace {
+
+struct Sub {
+    uint64_t count = 0, expansions = 0;
+};
+
+// Completions and loop expansions (one per queen placed, the last row included) below a prefix.
+void subtree(uint32_t last, uint32_t c, uint32_t l, uint32_t r, Sub *s) {
+    if (c == last) {
+        s->count++;
+        return;
+    }
+    uint32_t v = last & ~(c | l | r);
+    while (v) {
+        const uint32_t p = v & (0u - v);
+        v ^= p;
+        s->expansions++;
+        subtree(last, c | p, (l | p) << 1, (r | p) >> 1, s);
+    }
+}
+
+struct Lane {
+    bool has = true;
+    bool parked = true;  // v == 0 in the kernel: no root, or the root is exhausted
+    uint64_t work = 0;
+};
+
+struct Model {
+    uint32_t N, last, chunk, stack_depth;
+    const std::vector<uint32_t> *tasks;
+    uint64_t ntasks;
+    uint64_t cursor = 0;
+    std::vector<int> claimed;
+    uint64_t total = 0, flags = 0, roots = 0;
+    bool frames_ok = true;
+};
+
+template <int S>
+struct Wave {
+    Lane lane[64];
+    nq::TaskWalk<S> walk;
+    uint64_t next = 0, end = 0;
+    bool drained = false, finished = false;
+    uint64_t wsum = 0;
+
+    // One pass of the kernel's outer loop: refill, then trips until some lane parks.
+    void step(Model &m) {
+        uint64_t need = 0;
+        for (int i = 0; i < 64; ++i)
+            if (lane[i].has && lane[i].parked) need |= 1ull << i;
+        while (need) {
+            uint32_t dc = 0, dl = 0, dr = 0, dv = 0;
+            bool got = false;
+            while (!drained) {
+                uint64_t direct = 0;
+                got = walk.next(m.N, m.last, &dc, &dl, &dr, &dv, &direct);
+                wsum += direct;
+                if (got) break;
+                if (next >= end) {
+                    next = m.cursor;
+                    m.cursor += m.chunk;
+                    end = next + m.chunk;
+                }
+                if (next >= m.ntasks) {
+                    drained = true;
+                    break;
+                }
+                const uint64_t t = next++;
+                m.claimed[t]++;
+                const uint32_t c0 = (*m.tasks)[3 * t], l0 = (*m.tasks)[3 * t + 1], r0 = (*m.tasks)[3 * t + 2];
+                const uint32_t v0 = m.last & ~(c0 | l0 | r0);
+                const uint32_t d0 = __builtin_popcount(c0);
+                if (d0 + 1 >= m.N || v0 == 0) {
+                    wsum += d0 >= m.N ? (c0 == m.last ? 1u : 0u) : (d0 + 1 == m.N ? __builtin_popcount(v0) : 0u);
+                    continue;
+                }
+                if (d0 + S + 1 < m.N && m.N - 1 - (d0 + S) > m.stack_depth) {
+                    m.flags |= 1;
+                    continue;
+                }
+                walk.start(c0, l0, r0, v0, d0);
+            }
+            if (!got) {
+                for (int i = 0; i < 64; ++i)
+                    if ((need >> i) & 1) lane[i].has = false;
+                break;
+            }
+            const int sel = __builtin_ctzll(need);
+            need &= need - 1;
+            // The root dc|dl|dr with candidates dv: its DFS needs N-1-depth frames.
+            const uint32_t depth = __builtin_popcount(dc);
+            if (m.N - 1 - depth > m.stack_depth) m.frames_ok = false;
+            if (dv != (m.last & ~(dc | dl | dr)) || dv == 0) m.frames_ok = false;
+            Sub sub;
+            subtree(m.last, dc, dl, dr, &sub);
+            wsum += sub.count;
+            lane[sel].parked = false;
+            lane[sel].work = sub.expansions;
+            m.roots++;
+        }
+        bool live = false;
+        for (auto &ln : lane) live |= ln.has;
+        if (!live) {
+            finished = true;
+            m.total += wsum;
+            return;
+        }
+        // Trips of 16 iterations until some live lane is parked at a trip's end.
+        for (;;) {
+            bool any_parked = false;
+            for (auto &ln : lane) {
+                if (!ln.has) continue;
+                if (!ln.parked) {
+                    ln.work = ln.work > 16 ? ln.work - 16 : 0;
+                    if (ln.work == 0) ln.parked = true;
+                }
+                any_parked |= ln.parked;
+            }
+            if (any_parked) break;
+        }
+    }
+};
+
+template <int S>
+int run(Model &m, int nwaves) {
+    std::vector<Wave<S>> waves(nwaves);
+    // Waves take turns; a wave's turn count is bounded by its claims plus its roots, so a
+    // generous cap detects a wave that never terminates.
+    const uint64_t cap = 64 * (m.ntasks + 1) * 1000 + 1000000;
+    uint64_t turns = 0;
+    bool all = false;
+    while (!all && turns < cap) {
+        all = true;
+        for (auto &w : waves) {
+            if (w.finished) continue;
+            w.step(m);
+            all = false;
+            ++turns;
+        }
+    }
+    bool claims_ok = true;
+    for (int c : m.claimed) claims_ok &= c == 1;
+    printf("%llu %llu %llu %d %d %d\n", (unsigned long long)m.total, (unsigned long long)m.flags,
+           (unsigned long long)m.roots, claims_ok ? 1 : 0, m.frames_ok ? 1 : 0, all ? 1 : 0);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc != 7) {
+        fprintf(stderr, "usage: %s N split chunk waves stack_depth taskfile\n", argv[0]);
+        return 2;
+    }
+    Model m;
+    m.N = static_cast<uint32_t>(atoi(argv[1]));
+    const int split = atoi(argv[2]);
+    m.chunk = static_cast<uint32_t>(atoi(argv[3]));
+    const int nwaves = atoi(argv[4]);
+    m.stack_depth = static_cast<uint32_t>(atoi(argv[5]));
+    m.last = (1u << m.N) - 1u;
+    std::vector<uint32_t> tasks;
+    FILE *f = fopen(argv[6], "rb");
+    if (!f) return 2;
+    uint32_t w[3];
+    while (fread(w, sizeof w, 1, f) == 1) tasks.insert(tasks.end(), w, w + 3);
+    fclose(f);
+    m.tasks = &tasks;
+    m.ntasks = tasks.size() / 3;
+    m.claimed.assign(m.ntasks, 0);
+    switch (split) {
+        case 1: return run<1>(m, nwaves);
+        case 2: return run<2>(m, nwaves);
+        case 3: return run<3>(m, nwaves);
+        case 4: return run<4>(m, nwaves);
+    }
+    return 2;
+}
