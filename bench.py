@@ -30,10 +30,10 @@ METRIC = "N=21 wall s + DFS nodes/s on 1 GPU; batched-mix instances/s vs per-lau
 OPS_PER_NODE = 11          # p=v&-v (2), v^=p (1), c|=p (1), l=(l|p)<<1 (2), r=(r|p)>>1 (2), v'=last&~(c|l|r) (3)
 PEAK_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
 LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
-# HBM bytes per N=21 rows-6 launch of the DFS kernel (2 rows split on the device), from two
-# rocprofv3 --pmc passes scaled by a 256 MiB copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000):
-# profiles/dfs_kernel_n21_rows6_traffic.md.
-TRAFFIC_N21_ROWS6 = (4136125 + 533930) * 1024
+# HBM bytes per N=21 rows-6 launch of the DFS kernel (split_kernel, 2 rows walked in registers),
+# from two rocprofv3 --pmc passes scaled by a 256 MiB copy (tests/probes/copy_probe: FETCH_SIZE
+# x2.000, WRITE_SIZE x1.000): profiles/split_kernel_n21_rows6_traffic.md.
+TRAFFIC_N21_ROWS6 = (2 * 62428 + 2975) * 1024
 
 
 def log(msg: str) -> None:
@@ -174,6 +174,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=21)
     ap.add_argument("--rows", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-nodes", action="store_true", help="take the node count from the fixtures (profiling)")
     ap.add_argument("--no-wall", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
@@ -230,26 +231,34 @@ def main() -> int:
                         cfg=c or (cfg_nodes if count_nodes else cfg), count_nodes=count_nodes,
                         stream=stream.cuda_stream)
 
-    # Node count (node-counting kernel, once, untimed; not a warmup of the timed path).
-    d_result.zero_()
-    torch.cuda.synchronize()
-    run(True)
-    stream.synchronize()
-    res = d_result.cpu().numpy().view(np.uint64)
-    sum0, nodes = int(res[0]), int(res[1])
-    log(f"node-counting run: sum={sum0} total={2*sum0} nodes={nodes} flags={int(res[2])}")
-    if int(res[2]):
-        log("ERROR: device error flags set")
-        return 1
     with open(os.path.join(ROOT, "tests", "golden", "published.json")) as f:
         fixtures = json.load(f)
     want_nodes = fixtures.get("dfs_nodes", {}).get(f"{N},{rows}")
     want_total = fixtures["solutions"].get(str(N))
-    if (want_nodes is not None and nodes != want_nodes) or (rows >= 2 and want_total is not None
-                                                             and 2 * sum0 != want_total):
-        log(f"ERROR: nodes {nodes} / total {2 * sum0} differ from the fixtures {want_nodes} / {want_total}")
-        return 1
-    # Warmup: W untimed steps of exactly the timed work (they also allocate the split buffers).
+    if args.no_nodes:
+        # Profiling runs: the node count and total come from the fixtures (checked by the GPU tests
+        # and by every full bench run); the node-counting kernel is not run.
+        if want_nodes is None or want_total is None:
+            log("ERROR: --no-nodes needs fixture node and solution counts for this (N, rows)")
+            return 1
+        nodes, sum0 = int(want_nodes), int(want_total) // 2
+    else:
+        # Node count (node-counting kernel, once, untimed; not a warmup of the timed path).
+        d_result.zero_()
+        torch.cuda.synchronize()
+        run(True)
+        stream.synchronize()
+        res = d_result.cpu().numpy().view(np.uint64)
+        sum0, nodes = int(res[0]), int(res[1])
+        log(f"node-counting run: sum={sum0} total={2*sum0} nodes={nodes} flags={int(res[2])}")
+        if int(res[2]):
+            log("ERROR: device error flags set")
+            return 1
+        if (want_nodes is not None and nodes != want_nodes) or (rows >= 2 and want_total is not None
+                                                                 and 2 * sum0 != want_total):
+            log(f"ERROR: nodes {nodes} / total {2 * sum0} differ from the fixtures {want_nodes} / {want_total}")
+            return 1
+    # Warmup: W untimed steps of exactly the timed work.
     for i in range(args.warmup):
         d_result.zero_()
         run(False)

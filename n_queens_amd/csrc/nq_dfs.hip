@@ -47,6 +47,8 @@ namespace {
 constexpr uint32_t FLUSH_EVERY = 1u << 27;
 // Trips between folds of the wave-uniform 32-bit counter (64 lanes add to it).
 constexpr uint32_t FLUSH_EVERY_WAVE = 1u << 21;
+// split_kernel's guided claims: a wave takes 1 / (GUIDED_SHARES x waves) of the tasks left.
+constexpr uint64_t GUIDED_SHARES = 4;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
@@ -390,7 +392,8 @@ __device__ __forceinline__ uint32_t wave_leader() {
 }
 
 // Plain totals with the device split done in registers: each wave claims whole tasks from the
-// cursor (`chunk` at a time) and walks the top S rows of a task's subtree (TaskWalk, the same on
+// cursor (guided: a share of the remaining tasks, at least `chunk`) and walks the top S rows of a
+// task's subtree (TaskWalk, the same on
 // every lane: scalar registers); every node S rows below the task becomes one lane's DFS root
 // when that lane's previous root is exhausted.  The expanded list never exists in memory, the
 // launch needs no host synchronisation, and each task is read once.  A task's subtree stays on
@@ -423,6 +426,7 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
     nq::TaskWalk<S> walk;
     uint64_t next = 0, end = 0;
     bool drained = false;  // no task left to claim
+    const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (blockDim.x >> 6);
 
     for (;;) {
         // Lanes parked on the dead frame take the next roots, one each, lowest lane first.
@@ -436,12 +440,23 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
                 wsum += direct;
                 if (got) break;
                 if (next >= end) {
+                    // Guided claims: a share of the tasks left, 1 / (GUIDED_SHARES x waves) of them,
+                    // never less than `chunk`, so single tasks near the end.  The cursor is read
+                    // first (a wave that ran long would otherwise size its share from an old
+                    // reading).  Each cursor access is a device-scope atomic on one address (about
+                    // 100 B of memory traffic: the XCDs' L2s are not coherent), so there are few.
                     const uint32_t leader = wave_leader();
                     uint64_t g = 0;
-                    if (lane == leader) g = atomicAdd(&st->cursor, static_cast<unsigned long long>(chunk));
+                    if (lane == leader) {
+                        const uint64_t cur = __hip_atomic_load(&st->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const uint64_t left = ntasks > cur ? ntasks - cur : 0;
+                        const uint64_t want = max(static_cast<uint64_t>(chunk), left / (GUIDED_SHARES * nwaves));
+                        g = atomicAdd(&st->cursor, static_cast<unsigned long long>(want));
+                        end = g + want;  // the leader's copy; made uniform below
+                    }
                     g = first_lane_u64(g);
+                    end = first_lane_u64(end);
                     next = g;
-                    end = g + chunk;
                 }
                 if (next >= ntasks) {
                     drained = true;

@@ -27,7 +27,8 @@ def run_model(model, tmp_path, N, tasks, split, chunk, waves, stack):
     np.ascontiguousarray(tasks, dtype=np.uint32).tofile(f)
     out = subprocess.check_output([model, str(N), str(split), str(chunk), str(waves), str(stack), str(f)],
                                   timeout=120).split()
-    total, flags, roots, claims_ok, frames_ok, done = (int(x) for x in out)
+    total, flags, roots, claims_ok, frames_ok, done, cursor_claims = (int(x) for x in out)
+    run_model.cursor_claims = cursor_claims
     return total, flags, roots, claims_ok == 1, frames_ok == 1, done == 1
 
 
@@ -58,6 +59,16 @@ def test_walk_model_mixed_depths_and_edge_tasks(model, tmp_path):
         stack = N - 1 - split  # the empty board (depth 0) is the shallowest
         total, flags, _, claims, frames, done = run_model(model, tmp_path, N, tasks, split, 2, 5, stack)
         assert (claims, frames, done, flags, total) == (True, True, True, 0, want), split
+
+
+def test_walk_model_guided_claims_are_few(model, tmp_path):
+    """Guided claims: a wave takes a share of the tasks left, so a long list costs few cursor
+    atomics (each is device-wide traffic), and single tasks near the end keep the tail short."""
+    N, rows = 13, 5
+    tasks = oracle.gen_tasks(N, rows)  # 15,550 tasks
+    total, flags, _, claims, frames, done = run_model(model, tmp_path, N, tasks, 2, 1, 7, N - 1 - rows - 2)
+    assert (claims, frames, done, flags, total) == (True, True, True, 0, oracle.count_tasks(N, tasks))
+    assert run_model.cursor_claims < len(tasks) // 20, run_model.cursor_claims
 
 
 def test_walk_model_flags_a_stack_too_small(model, tmp_path):

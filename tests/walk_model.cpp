@@ -1,14 +1,15 @@
 // CPU model of split_kernel's control flow (n_queens_amd/csrc/nq_dfs.hip), built around the
 // kernel's own TaskWalk (n_queens_amd/csrc/nq_walk.h, compiled here for the host).
 //
-// Waves share one task cursor and claim `chunk` tasks at a time; each wave walks its task's top
-// `split` rows and hands every root to its lowest lane that is parked (no root, or the last one
-// exhausted), exactly as the kernel's refill loop does.  A lane's DFS is modelled by its work:
+// Waves share one task cursor and claim guided shares of the tasks left (at least `chunk`); each
+// wave walks its task's top `split` rows and hands every root to its lowest lane that is parked
+// (no root, or the last one exhausted), exactly as the kernel's refill loop does.  A lane's DFS is modelled by its work:
 // the root's expansions, consumed 16 per trip, and its count, the root's completions (both from
 // a plain recursion).  Waves advance round-robin, one refill + trips-until-a-lane-parks at a
 // time, so lanes finish at different times and waves interleave their claims.
 //
-// Checks, reported on stdout as one line "total flags roots claims_ok frames_ok terminated":
+// Checks, reported on stdout as one line "total flags roots claims_ok frames_ok terminated
+// cursor_claims":
 //   * every task index is claimed exactly once;
 //   * every handed-out root needs at most stack_depth frames (N - 1 - its depth);
 //   * every wave terminates.
@@ -54,7 +55,7 @@ struct Model {
     uint32_t N, last, chunk, stack_depth;
     const std::vector<uint32_t> *tasks;
     uint64_t ntasks;
-    uint64_t cursor = 0;
+    uint64_t cursor = 0, nwaves = 1, claims = 0;
     std::vector<int> claimed;
     uint64_t total = 0, flags = 0, roots = 0;
     bool frames_ok = true;
@@ -82,9 +83,15 @@ struct Wave {
                 wsum += direct;
                 if (got) break;
                 if (next >= end) {
+                    // split_kernel's guided claim: 1 / (GUIDED_SHARES = 4 x waves) of the tasks
+                    // left at the current cursor, at least `chunk`.
+                    const uint64_t left = m.ntasks > m.cursor ? m.ntasks - m.cursor : 0;
+                    uint64_t want = left / (4 * m.nwaves);
+                    if (want < m.chunk) want = m.chunk;
                     next = m.cursor;
-                    m.cursor += m.chunk;
-                    end = next + m.chunk;
+                    m.cursor += want;
+                    end = next + want;
+                    m.claims++;
                 }
                 if (next >= m.ntasks) {
                     drained = true;
@@ -149,6 +156,7 @@ struct Wave {
 template <int S>
 int run(Model &m, int nwaves) {
     std::vector<Wave<S>> waves(nwaves);
+    m.nwaves = static_cast<uint64_t>(nwaves);
     // Waves take turns; a wave's turn count is bounded by its claims plus its roots, so a
     // generous cap detects a wave that never terminates.
     const uint64_t cap = 64 * (m.ntasks + 1) * 1000 + 1000000;
@@ -165,8 +173,9 @@ int run(Model &m, int nwaves) {
     }
     bool claims_ok = true;
     for (int c : m.claimed) claims_ok &= c == 1;
-    printf("%llu %llu %llu %d %d %d\n", (unsigned long long)m.total, (unsigned long long)m.flags,
-           (unsigned long long)m.roots, claims_ok ? 1 : 0, m.frames_ok ? 1 : 0, all ? 1 : 0);
+    printf("%llu %llu %llu %d %d %d %llu\n", (unsigned long long)m.total, (unsigned long long)m.flags,
+           (unsigned long long)m.roots, claims_ok ? 1 : 0, m.frames_ok ? 1 : 0, all ? 1 : 0,
+           (unsigned long long)m.claims);
     return 0;
 }
 
