@@ -231,6 +231,19 @@ int nq_server_timing(nq_server *server, double *device_ms, double *open_ms);
 /* Finish every admitted request, stop the scheduler and free the device queues. */
 int nq_server_close(nq_server *server);
 
+/* ---- counting without device allocation (no reference counterpart) ---------------------
+ * For callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc): nq_count's counts (sum, per_task, nodes, same cfg rules) with every device buffer
+ * nq_count would allocate — the device state, the result words and the staging of the task list
+ * and of the per-task counts — in one static device array of NQ_STATIC_SCRATCH_BYTES (about 20
+ * MiB) that each device the library runs on holds until the process ends.  At most
+ * NQ_STATIC_MAX_TASKS tasks per call (NQ_ERR_ARG beyond).  Calls on one device share its array
+ * and are serialised by the library; each call resets what must start at zero. */
+#define NQ_STATIC_MAX_TASKS (1u << 20)
+#define NQ_STATIC_SCRATCH_BYTES (256u + 20u * NQ_STATIC_MAX_TASKS)
+int nq_count_static(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg, uint64_t *sum,
+                    uint64_t *per_task, uint64_t *nodes);
+
 /* ---- resumable counts (no reference counterpart) --------------------------------------
  * For runs that take days (the reference's N=27 run took 28.4 days with no way to resume,
  * README.md:46).  The (N, rows) task list is the host generator's (reference order), counted

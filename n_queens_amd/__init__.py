@@ -39,8 +39,9 @@ EXPORTS = (
     "nq_batch_plan_create", "nq_batch_plan_launch", "nq_batch_plan_wait", "nq_batch_plan_run",
     "nq_batch_plan_destroy", "nq_batch_count", "nq_instance_count",
     "nq_server_config_default", "nq_server_open", "nq_server_submit", "nq_server_poll", "nq_server_wait",
-    "nq_server_stats", "nq_server_timing", "nq_server_close", "nq_solve_resumable",
+    "nq_server_stats", "nq_server_timing", "nq_server_close", "nq_solve_resumable", "nq_count_static",
 )
+NQ_STATIC_MAX_TASKS = 1 << 20
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -155,6 +156,8 @@ def lib() -> ctypes.CDLL:
     L.nq_server_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.nq_solve_resumable.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Config), ctypes.c_char_p,
                                      ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_ResumeState)]
+    L.nq_count_static.argtypes = [ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.POINTER(_Config), _u64p, _u64p,
+                                  _u64p]
     for name in EXPORTS:
         if name not in ("nq_last_error",):
             getattr(L, name).restype = ctypes.c_int
@@ -241,6 +244,21 @@ def count(N: int, tasks: np.ndarray, cfg: Config | None = None, per_task: bool =
                           ctypes.byref(st)))
     return CountResult(int(s.value), pt[:n] if per_task else None, int(nd.value) if nodes else None,
                        st.kernel_ms, st.h2d_ms, st.total_ms, st.busy_frac, st.tail_ms, int(st.waves))
+
+
+def count_static(N: int, tasks: np.ndarray, cfg: Config | None = None, per_task: bool = False,
+                 nodes: bool = False) -> CountResult:
+    """nq_count_static: count() with no device allocation (at most NQ_STATIC_MAX_TASKS tasks)."""
+    t = np.ascontiguousarray(tasks, dtype=np.uint32).reshape(-1, 3)
+    n = t.shape[0]
+    s = ctypes.c_uint64()
+    nd = ctypes.c_uint64()
+    pt = np.zeros(max(n, 1), dtype=np.uint64) if per_task else None
+    cc = cfg._c() if cfg else None
+    _check(lib().nq_count_static(N, t.ctypes.data_as(_u32p), n, ctypes.byref(cc) if cc else None, ctypes.byref(s),
+                                 pt.ctypes.data_as(_u64p) if per_task else None,
+                                 ctypes.byref(nd) if nodes else None))
+    return CountResult(int(s.value), pt[:n] if per_task else None, int(nd.value) if nodes else None, 0.0, 0.0, 0.0)
 
 
 def count_device(N: int, d_tasks: int, ntasks: int, d_result: int, d_per_task: int = 0, cfg: Config | None = None,

@@ -29,7 +29,7 @@ HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + CSRC]
 
-LIB_SOURCES = ["nq_dfs.hip", "nq_expand.hip", "nq_api.cpp", "nq_prefix.cpp", "nq_compat.cpp", "nq_batch.cpp", "nq_serve.cpp",
+LIB_SOURCES = ["nq_dfs.hip", "nq_expand.hip", "nq_static.hip", "nq_api.cpp", "nq_prefix.cpp", "nq_compat.cpp", "nq_batch.cpp", "nq_serve.cpp",
                "nq_resume.cpp", "nq_cpu.cpp"]
 
 
