@@ -52,6 +52,9 @@ typedef struct nq_config {
     int push;         /* DFS push form: NQ_PUSH_AUTO (0) takes NQ_PUSH_ALWAYS when its spare */
                       /* stack level costs no resident waves, else NQ_PUSH_MASKED.  Both    */
                       /* give identical counts.                                              */
+    int preplace;     /* nq_solve only: depth of the task list the waves claim, generated on */
+                      /* the device (0 or <= rows: rows itself); the reference's task count  */
+                      /* for `rows` is still the one reported.                               */
 } nq_config;
 
 #define NQ_PUSH_AUTO 0
@@ -87,11 +90,15 @@ int nq_device_info(int device, int *num_cus, int *clock_khz, char *name, size_t 
 int nq_config_default(int N, int d0, nq_config *cfg);
 
 /* The configuration nq_solve(N, rows, NULL, ...) uses, in the form nq_solve and nq_count take
- * it back: stack_depth = N - 1 - rows bounds the depth-`rows` tasks (the count takes the split
- * rows off it), split = the automatic device split of clamp(N - rows - R, 0, 3) rows, so that
- * the DFS starts R rows above the last row (R = (N+4)/2 up to N = 20, R = 13 from N = 21 on:
- * 12 LDS frames), and chunk 1 with a split.  nq_solve(N, rows, &cfg, ...) with this cfg is
- * nq_solve(N, rows, NULL). */
+ * it back.  When the autotuner's cache (python -m n_queens_amd.tune; the file $NQ_TUNE_CACHE,
+ * else $HOME/.cache/n_queens_amd/tune.txt) holds an entry for N on the current device: its
+ * preplace depth, split, block, chunk, push, waves per CU and stack depth, with a preplace depth
+ * below `rows` raised to `rows` and the split shortened to keep the tuned DFS depth.  Otherwise
+ * the built-in rule: preplace 0 (= rows), split = clamp(N - rows - R, 0, 3) so that the DFS
+ * starts R rows above the last row (R = (N+4)/2 up to N = 20, R = 13 from N = 21 on: 12 LDS
+ * frames), chunk 1 with a split.  stack_depth bounds the claimed tasks (depth
+ * max(rows, preplace)); the count takes the split rows off it.  No device is needed for the
+ * built-in rule.  nq_solve(N, rows, &cfg, ...) with this cfg is nq_solve(N, rows, NULL). */
 int nq_solve_config(int N, int rows, nq_config *cfg);
 
 /* The launch configuration nq_count takes on the current device for ntasks tasks whose

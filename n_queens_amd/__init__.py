@@ -55,7 +55,8 @@ class NQError(RuntimeError):
 
 class _Config(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("stack_depth", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
-                ("chunk", ctypes.c_int), ("grid", ctypes.c_int), ("split", ctypes.c_int), ("push", ctypes.c_int)]
+                ("chunk", ctypes.c_int), ("grid", ctypes.c_int), ("split", ctypes.c_int), ("push", ctypes.c_int),
+                ("preplace", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -101,14 +102,15 @@ class Config:
     grid: int = 0
     split: int = -1
     push: int = 0
+    preplace: int = 0
 
     def _c(self) -> _Config:
         return _Config(self.block, self.stack_depth, self.waves_per_cu, self.chunk, self.grid, self.split,
-                       self.push)
+                       self.push, self.preplace)
 
     @staticmethod
     def _from(c: _Config) -> "Config":
-        return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push)
+        return Config(c.block, c.stack_depth, c.waves_per_cu, c.chunk, c.grid, c.split, c.push, c.preplace)
 
 
 _lib = None

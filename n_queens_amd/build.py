@@ -30,7 +30,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + CSRC]
 
 LIB_SOURCES = ["nq_dfs.hip", "nq_expand.hip", "nq_static.hip", "nq_api.cpp", "nq_prefix.cpp", "nq_compat.cpp", "nq_batch.cpp", "nq_serve.cpp",
-               "nq_resume.cpp", "nq_cpu.cpp"]
+               "nq_resume.cpp", "nq_cpu.cpp", "nq_tune.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

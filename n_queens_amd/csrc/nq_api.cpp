@@ -103,6 +103,16 @@ static int current_device(Device **out) {
     return NQ_OK;
 }
 
+// gcnArchName of the current device (the tuner cache's key); false without a device.
+static bool current_arch(char *out, size_t len) {
+    int n = 0, id = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || hipGetDevice(&id) != hipSuccess) return false;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, id) != hipSuccess) return false;
+    snprintf(out, len, "%s", prop.gcnArchName);
+    return out[0] != 0;
+}
+
 static int check_cfg_shape(const nq_config &c) {
     if (c.block != 0 && (c.block < 64 || c.block > MAX_BLOCK || c.block % 64))
         return fail(NQ_ERR_ARG, "block must be 0 (automatic) or 64..256, multiple of 64");
@@ -368,8 +378,14 @@ int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_s
     std::vector<uint32_t> seeds;
     generate_tasks(N, seed_rows, seeds);
     const uint64_t nseeds = seeds.size() / 3;
+    // Without a configuration: the tuned one for N on this device if the tuner's cache has it,
+    // else the built-in rule (nq_solve_config).
     nq_config in = cfg ? *cfg : nq_config{};
-    if (!cfg) in.split = -1;
+    if (!cfg) {
+        rc = nq_solve_config(N, rows, &in);
+        if (rc) return rc;
+    }
+    const int preplace = std::min(N, std::max(rows, in.preplace));
 
     std::lock_guard<std::mutex> g(d->mu);
     rc = ensure_tasks(d, nseeds);
@@ -395,8 +411,17 @@ int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_s
         if (rc) return rc;
     }
     NQ_HIP(hipStreamSynchronize(d->stream));
-    const double t3 = now_ms();
     const uint64_t ntasks_rows = n;  // the reference's task count for (N, rows)
+    // Deeper claimed tasks when the configuration asks for them (the tuner's preplace depth).
+    if (nseeds && preplace > seed_rows + gen) {
+        int more = 0;
+        rc = enqueue_split(d, N, seed_rows + gen, t, n_dev, n, preplace - (seed_rows + gen), gen & 1, true, d->stream,
+                           &t, &n_dev, &n, &more);
+        if (rc) return rc;
+        gen += more;
+        NQ_HIP(hipStreamSynchronize(d->stream));
+    }
+    const double t3 = now_ms();
     if (on_generated) on_generated(ctx, ntasks_rows, t3 - t0 - (t2 - t1));
 
     // Count, the kernel walking `split` rows below every task in registers.  cfg->stack_depth
@@ -631,11 +656,25 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
 int nq_solve_config(int N, int rows, nq_config *cfg) {
     if (!cfg) return fail(NQ_ERR_ARG, "cfg is NULL");
     if (N < 1 || N > MAX_N || rows < 1 || rows > N) return fail(NQ_ERR_ARG, "N must be 1..28 and rows 1..N");
-    // stack_depth bounds the depth-`rows` tasks, as nq_solve and nq_count read it (they take the
-    // split rows off it for the DFS after the split).
+    // stack_depth bounds the claimed tasks, as nq_solve and nq_count read it (they take the split
+    // rows off it for the DFS after the split).
     nq_config_default(N, rows, cfg);
     cfg->split = auto_split(N, rows);
     if (cfg->split > 0) cfg->chunk = 1;  // whole tasks per wave claim
+    cfg->preplace = 0;
+    char arch[256];
+    TunedEntry e;
+    if (current_arch(arch, sizeof arch) && find_tuned(arch, N, &e)) {
+        // The tuned DFS depth (preplace + split), reached from at least `rows`.
+        const int g = std::min(N, std::max(rows, e.preplace));
+        cfg->preplace = g > rows ? g : 0;
+        cfg->split = std::max(0, std::min(MAX_SPLIT, e.preplace + e.split - g));
+        cfg->block = e.block;
+        cfg->chunk = e.chunk ? e.chunk : (cfg->split > 0 ? 1 : 16);
+        cfg->push = e.push;
+        cfg->waves_per_cu = e.waves_per_cu;
+        cfg->stack_depth = std::max(1, frames_needed(N, g) + e.stack_extra);
+    }
     return NQ_OK;
 }
 

@@ -32,6 +32,18 @@ int plan_count(int cus, int N, int d0, uint64_t ntasks, const nq_config *cfg, bo
 // Shallowest prefix depth (queens placed) over a task list.
 int task_min_depth(int N, const uint32_t *tasks, uint64_t ntasks);
 
+// One entry of the autotuner's cache (nq_tune.cpp): the fastest DFS launch for whole boards of
+// size N on one device type.
+struct TunedEntry {
+    int N = 0, preplace = 0, split = 0, block = 0, chunk = 0, push = 0, waves_per_cu = 0, stack_extra = 0;
+};
+
+// The cache file's path ($NQ_TUNE_CACHE, else $HOME/.cache/n_queens_amd/tune.txt; "" = none).
+std::string tune_cache_path();
+
+// The last cache entry for (device arch, N), if any.
+bool find_tuned(const char *arch, int N, TunedEntry *out);
+
 // Thread-local error reporting for the C ABI.
 int fail(int status, const std::string &msg);
 int fail_hip(int hip_error, const char *what);

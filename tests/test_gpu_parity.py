@@ -257,17 +257,6 @@ def _random_prefix(rng, N, k):
     return cols
 
 
-def test_tuner_small_board(tmp_path, published):
-    """The autotuner times every candidate, agrees on the total and caches the winner."""
-    from n_queens_amd import tune
-    path = str(tmp_path / "tune.json")
-    win, res = tune.tune(13, 3, quick=True, path=path, log=lambda s: None)
-    assert len(res) == len(tune.candidates(13, 3, quick=True))
-    assert tune.tuned_config(13, 3, path=path) == win
-    total, _ = nq.solve(13, 3, win)
-    assert total == published["solutions"][13]
-
-
 @pytest.mark.parametrize("N,rows", [(8, 3), (9, 4), (12, 2), (13, 5), (14, 7), (15, 1), (16, 6), (11, 9),
                                     (10, 10), (17, 3)])
 def test_device_generation_matches_host_generator(N, rows):

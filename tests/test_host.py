@@ -207,15 +207,39 @@ def test_cli_no_gpu_path():
     assert re.search(r"cuda 13 queens result -1, calc time: \[[\d.]+ms\]", p.stdout)
 
 
-def test_tuner_candidates_respect_depth():
-    """The autotuner only proposes splits that leave the DFS two rows to place."""
+def test_tuner_search_space():
+    """The autotuner's phases: preplace 5..7 x split 0..4 keeping three rows for the DFS, then
+    workgroup size x push form, chunk 1..64, waves per CU 4..16 and extra LDS frames."""
     from n_queens_amd import tune
-    for N, rows in ((21, 6), (12, 9), (8, 5), (6, 5)):
-        cands = tune.candidates(N, rows)
-        assert cands
-        assert all(rows + c.split + 2 < N or c.split == 0 for c in cands)
-        assert {c.push for c in cands} == {nq.NQ_PUSH_MASKED, nq.NQ_PUSH_ALWAYS}
-    assert len(tune.candidates(21, 6, quick=True)) == 8
+    for N in (21, 20, 12, 9):
+        pts = tune.depth_points(N)
+        assert pts and all(2 <= p.preplace and p.preplace + p.split <= N - 3 for p in pts), N
+    assert {(p.preplace, p.split) for p in tune.depth_points(21)} == {(g, s) for g in (5, 6, 7) for s in range(5)}
+    p = tune.Point(6, 2)
+    assert {(q.block, q.push) for q in tune.launch_points(p)} == {(b, k) for b in (64, 128, 256) for k in (1, 2)}
+    assert [q.chunk for q in tune.chunk_points(p)] == [1, 4, 16, 64]
+    occ = tune.occupancy_points(p)
+    assert [q.waves_per_cu for q in occ[:4]] == [4, 8, 12, 16] and [q.stack_extra for q in occ[4:]] == [1, 2]
+    # The configuration timed for a point: stack depth for the depth-`preplace` tasks (the count
+    # takes the split rows off it).
+    c = tune.Point(6, 2, block=256, chunk=4, push=2, waves_per_cu=12, stack_extra=1).config(21)
+    assert (c.stack_depth, c.split, c.block, c.chunk, c.push, c.waves_per_cu) == (15, 2, 256, 4, 2, 12)
+
+
+def test_tuner_cache_file_format(tmp_path):
+    """The cache is plain text the C library parses (nq_tune.cpp): the last line for (arch, N)
+    wins, comments and malformed lines are skipped."""
+    from n_queens_amd import tune
+    p = tmp_path / "t.txt"
+    p.write_text("# comment\n"
+                 "gfx950:sramecc+:xnack- N=21 preplace=6 split=2 block=256 chunk=1 push=2 waves_per_cu=0 "
+                 "stack_extra=0 ms=7345.1\n"
+                 "gfx950:sramecc+:xnack- N=21 preplace=5 split=3 block=128 chunk=4 push=1 waves_per_cu=12 "
+                 "stack_extra=1 ms=7300.0\n"
+                 "gfx950:sramecc+:xnack- N=20 preplace=6 broken\n")
+    c = tune.load_cache(str(p))
+    assert c[("gfx950:sramecc+:xnack-", 21)]["preplace"] == 5 and c[("gfx950:sramecc+:xnack-", 21)]["ms"] == 7300.0
+    assert ("gfx950:sramecc+:xnack-", 20) not in c or "split" not in c[("gfx950:sramecc+:xnack-", 20)]
 
 
 def _write(path, text):
@@ -261,15 +285,6 @@ except nq.NQError as e:
     assert open(p).read() == "nq-checkpoint 1\nN 13 rows 5 ntasks 15550 next 0 sum 0\n"
 
 
-def test_tuner_occupancy_candidates():
-    from n_queens_amd import tune
-    win = nq.Config(block=128, chunk=16, split=2)
-    occ = tune.occupancy_candidates(20, 6, win)
-    assert [c.waves_per_cu for c in occ[:4]] == [4, 8, 12, 16]
-    assert [c.stack_depth for c in occ[4:]] == [13, 15, 17]
-    assert all(c.split == 2 and c.block == 128 for c in occ)
-
-
 def test_default_and_solve_configs_leave_block_automatic():
     """nq_config_default / nq_solve_config need no device: workgroup size 0 (automatic, chosen by
     occupancy at launch), the smallest legal stack, chunk 16, automatic push, and the split rule
@@ -292,3 +307,36 @@ except nq.NQError as e:
 """], capture_output=True, text=True, env=NO_GPU, cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
         timeout=120)
     assert f"status {nq.NQ_ERR_NO_DEVICE}" in r.stdout, r.stdout + r.stderr
+
+
+def test_tuner_cache_parsed_by_the_library_reader(tmp_path):
+    """The C reader of the tuner's cache (n_queens_amd/csrc/nq_tune.cpp), built on the host with a
+    small driver: the last entry for (arch, N) wins, timing fields are ignored, comments and
+    malformed or out-of-range entries are skipped, and NQ_TUNE_CACHE="" switches the cache off."""
+    drv = tmp_path / "drv.cpp"
+    drv.write_text('#include <cstdio>\n#include <cstdlib>\n#include "nq_internal.h"\n'
+                   "int main(int c, char **v) { nq::TunedEntry e; bool ok = nq::find_tuned(v[1], atoi(v[2]), &e);\n"
+                   ' printf("%d %d %d %d %d %d %d %d %d\\n", ok, e.N, e.preplace, e.split, e.block, e.chunk, e.push,'
+                   " e.waves_per_cu, e.stack_extra); }\n")
+    exe = tmp_path / "drv"
+    subprocess.check_call(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I",
+                           os.path.join(ROOT, "n_queens_amd", "csrc"), str(drv),
+                           os.path.join(ROOT, "n_queens_amd", "csrc", "nq_tune.cpp"), "-o", str(exe)])
+    cache = tmp_path / "t.txt"
+    cache.write_text("# tuned on one device\n"
+                     "gfx950:a N=17 preplace=7 split=2 block=64 chunk=4 push=1 waves_per_cu=8 stack_extra=1 ms=3.5\n"
+                     "gfx950:a N=21 preplace=6 split=2 block=256 chunk=1 push=2 waves_per_cu=0 stack_extra=0\n"
+                     "gfx950:a N=21 preplace=7 split=1 block=0 chunk=0 push=0 waves_per_cu=0 stack_extra=0 ms=7336.0\n"
+                     "gfx950:a N=16 preplace=x split=2\n"
+                     "gfx950:a N=15 preplace=5 split=9 block=0 chunk=0 push=0 waves_per_cu=0 stack_extra=0\n"
+                     "gfx942:b N=17 preplace=5 split=1 block=0 chunk=0 push=0 waves_per_cu=0 stack_extra=0\n")
+
+    def look(arch, N, path=str(cache)):
+        out = subprocess.check_output([str(exe), arch, str(N)], env=dict(os.environ, NQ_TUNE_CACHE=path))
+        return [int(x) for x in out.split()]
+
+    assert look("gfx950:a", 17) == [1, 17, 7, 2, 64, 4, 1, 8, 1]
+    assert look("gfx950:a", 21) == [1, 21, 7, 1, 0, 0, 0, 0, 0]   # the last line wins
+    assert look("gfx950:a", 16)[0] == 0 and look("gfx950:a", 15)[0] == 0  # malformed, split > 4
+    assert look("gfx942:b", 17)[:4] == [1, 17, 5, 1] and look("gfx950:a", 18)[0] == 0
+    assert look("gfx950:a", 17, path="")[0] == 0
