@@ -11,6 +11,8 @@ allowed.
 """
 from __future__ import annotations
 
+import hashlib
+
 import numpy as np
 
 BATCH_SEED = 0x6E51
@@ -52,3 +54,10 @@ def poisson_requests(n: int, lam: float, seed: int = 0x5E7E, n_range=(12, 19), k
         k = int(rng.integers(k_range[0], k_range[1] + 1))
         out.append((t, N, random_prefix(rng, N, k)))
     return out
+
+
+def instances_digest(insts) -> str:
+    """A short digest of an instance list [(N, cols), ...]: tells whether a stored per-instance
+    result (tests/golden/instance_counts.json) was made for exactly these instances."""
+    text = ";".join(f"{N}:{','.join(map(str, cols))}" for N, cols in insts)
+    return hashlib.sha256(text.encode()).hexdigest()[:16]
