@@ -111,14 +111,18 @@ def serving(nq, log, n_req: int = 400, util: float = 0.7) -> dict:
     alone = [nq.instance_count(N, cols) for _, N, cols in reqs0]
     busy = [0]
 
-    def submit(srv, N, cols):
-        # The server refuses work beyond its pending-task budget (NQ_ERR_BUSY): back off, retry.
+    def submit(srv, N, cols, deadline_s: float = 60.0):
+        # The server refuses work beyond its pending-task budget (NQ_ERR_BUSY): back off and retry,
+        # for at most deadline_s (a request that stays refused that long is an error).
+        t_end = time.perf_counter() + deadline_s
         while True:
             try:
                 return srv.submit(N, cols)
             except nq.NQError as e:
                 if e.status != nq.NQ_ERR_BUSY:
                     raise
+                if time.perf_counter() > t_end:
+                    raise RuntimeError(f"request N={N} {cols} refused as busy for {deadline_s} s") from e
                 busy[0] += 1
                 time.sleep(0.0005)
 

@@ -192,28 +192,28 @@ int nq_batch_count(const nq_instance *inst, int n, const nq_config *cfg, uint64_
 int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *count);
 
 /* ---- serving with continuous batching (no reference counterpart) ---------------------
- * A server keeps `slots` device-resident task queues (tasks, per-request counters, cursor,
- * pinned result buffer, stream).  nq_server_submit validates a request, expands it into tasks
- * on the caller's thread and admits it while the pending work fits max_pending_tasks (else
- * NQ_ERR_BUSY, which a later retry may pass; a request larger than slot_tasks can never fit
- * and is refused with NQ_ERR_NOMEM).  A request is
- * expanded until no task keeps more than depth_target free rows.  A scheduler thread
- * cuts a batch into each free slot: pending requests by least estimated remaining work (aged
- * by waiting time), each contributing tasks until the batch's estimated work (batch_work DFS
- * nodes) is spent, so large requests are served in slices over consecutive batches while small
- * ones join the next batch; batches run on their slot's stream and overlap on the device.  A
+ * A server keeps one device-resident ring of tasks that a persistent kernel consumes while the
+ * server keeps appending to it.  nq_server_submit validates a request, expands it into tasks on
+ * the caller's thread (until no task keeps more than depth_target free rows) and admits it
+ * while the pending work fits max_pending_tasks (else NQ_ERR_BUSY, which a later retry may
+ * pass; a request larger than the ring can never fit and is refused with NQ_ERR_NOMEM).  A
+ * scheduler thread publishes requests into the ring in slices of at most batch_work estimated
+ * DFS nodes, least estimated remaining work first (aged by waiting time), keeping no more than
+ * two slices' work published and unclaimed, so that requests arriving while others run are
+ * admitted into the running kernel within that much work.  The kernel counts every task into
+ * its request's slot and publishes a request's count as soon as its last task is counted.  A
  * request's count equals nq_instance_count's for the same instance.  Calls on one server are
  * thread-safe. */
 typedef struct nq_server_config {
-    int slots;                   /* concurrent batches in flight (default 3)               */
-    uint64_t slot_tasks;         /* task capacity of one slot (default 16M)                */
-    int slot_requests;           /* requests per batch (default 4096)                      */
+    int slots;                   /* streams for overlapping kernel instances (default 4)    */
+    uint64_t slot_tasks;         /* ring capacity in tasks (default 32M; rounded up to 2^k)  */
+    int slot_requests;           /* published slices not yet counted (default 4096)          */
     uint64_t max_pending_tasks;  /* admission budget of queued tasks (default 64M)         */
     int depth_target;            /* most free rows a task keeps (default 12): bounds both  */
                                  /* the LDS stack and one task's run time on its lane      */
     int block;                   /* workgroup size (default 128)                           */
-    int chunk;                   /* tasks per cursor grab (default 16)                     */
-    double batch_work;           /* estimated DFS nodes per batch (default 1e10, ~5 ms)    */
+    int chunk;                   /* tasks per claim of a wave (default 16)                 */
+    double batch_work;           /* estimated DFS nodes per published slice (default 1e10) */
 } nq_server_config;
 
 typedef struct nq_request_stats {
@@ -232,8 +232,10 @@ int nq_server_poll(nq_server *server, uint64_t ticket, uint64_t *count, nq_reque
 /* Block until done or timeout_ms (< 0: no timeout); NQ_ERR_STATE on timeout. */
 int nq_server_wait(nq_server *server, uint64_t ticket, int timeout_ms, uint64_t *count,
                    nq_request_stats *stats);
+/* batches: slices published into the ring. */
 int nq_server_stats(nq_server *server, uint64_t *completed, uint64_t *batches, uint64_t *solutions);
-/* Summed batch residency (launch -> results observed on the host) and time since open. */
+/* Time with at least one request in the ring (its first slice published -> count observed on
+ * the host), and time since open. */
 int nq_server_timing(nq_server *server, double *device_ms, double *open_ms);
 /* Finish every admitted request, stop the scheduler and free the device queues. */
 int nq_server_close(nq_server *server);

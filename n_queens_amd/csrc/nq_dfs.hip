@@ -214,8 +214,10 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 template <bool NODES, bool APUSH, bool WAVE>
 __device__ __forceinline__ void hot_loop(uint32_t &c, uint32_t &l, uint32_t &r, uint32_t &v, uint32_t &sp,
                                          uint32_t &cnt, uint32_t &wcnt, uint32_t &nd, uint64_t live,
-                                         uint32_t stride) {
-    uint32_t t, p, bc, iters = WAVE ? FLUSH_EVERY_WAVE : FLUSH_EVERY;
+                                         uint32_t stride, uint32_t max_trips = 0) {
+    // At most the fold interval of the counters (a smaller bound returns sooner).
+    const uint32_t fold = WAVE ? FLUSH_EVERY_WAVE : FLUSH_EVERY;
+    uint32_t t, p, bc, iters = (max_trips && max_trips < fold) ? max_trips : fold;
     uint64_t m1, save;
     if (WAVE) {
         static_assert(!WAVE || !NODES, "the node-counting variant counts per lane");
@@ -615,6 +617,207 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
     }
 }
 
+// ---- serving: persistent kernel over a ring the host appends to (ServeArgs in nq_kernels.h) ----
+
+// Trips an idle lane's wave keeps running its busy lanes before it looks for new tasks.
+constexpr uint32_t SERVE_IDLE_TRIPS = 64;
+// Compare-and-swap attempts per claim (a failed round leaves the lanes parked until the next one).
+constexpr int SERVE_CAS_TRIES = 64;
+
+__device__ __forceinline__ uint64_t sys_load_u64(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// A ring entry, read past the caches: the host's copy engine wrote it after the kernel started.
+__device__ __forceinline__ uint4 load_ring(const uint4 *p) {
+    const unsigned int *w = reinterpret_cast<const unsigned int *>(p);
+    return make_uint4(__hip_atomic_load(w + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                      __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                      __hip_atomic_load(w + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                      __hip_atomic_load(w + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+}
+
+// Claim one published task index for every active lane (claim() with the global cursor bounded
+// by the host's tail): lanes take from the wave's LDS range first; the first active lane then
+// takes max(chunk, lanes left) more with a compare-and-swap on head, never past tail.  *got is
+// false for lanes that found nothing published (they stay parked).
+__device__ __forceinline__ uint64_t serve_claim(unsigned long long *wq_, const ServeArgs &a, uint32_t chunk, bool *got) {
+    volatile __attribute__((address_space(3))) unsigned long long *wq =
+        (volatile __attribute__((address_space(3))) unsigned long long *)wq_;
+    const uint64_t mask = __ballot(1);
+    const uint32_t k = __popcll(mask);
+    const uint32_t rank = lane_rank(mask);
+    const uint64_t next = wq[0];
+    const uint64_t end = wq[1];
+    const uint64_t avail = end - next;
+    if (avail >= k) {
+        if (rank == 0) wq[0] = next + k;
+        *got = true;
+        return next + rank;
+    }
+    uint64_t base = 0, n = 0;
+    // An instance retires once one of its waves has exited (ServeCtl::exited at or past its
+    // epoch): its waves then take no new tasks, finish the ones they hold and exit, so that a
+    // fresh instance with every wave takes the work from there.
+    const uint32_t exited = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(&a.ctl->exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+    if (rank == 0 && static_cast<int>(exited - a.epoch) < 0) {
+        const uint64_t want = max(k - avail, static_cast<uint64_t>(chunk));
+        const uint64_t tail = sys_load_u64(&a.ctl->tail);
+        unsigned long long h = __hip_atomic_load(a.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int tries = 0; tries < SERVE_CAS_TRIES && h < tail; ++tries) {
+            const unsigned long long take = min(want, tail - h);
+            if (__hip_atomic_compare_exchange_strong(a.head, &h, h + take, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+                base = h;
+                n = take;
+                // Progress for the host's scheduler (plain store: stores of two waves may land
+                // out of order, the host only uses it to size how much it publishes ahead).
+                __hip_atomic_store(&a.ctl->claimed, h + take, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+        }
+    }
+    base = first_lane_u64(base);
+    n = first_lane_u64(n);
+    uint64_t mine = 0;
+    *got = true;
+    if (rank < avail)
+        mine = next + rank;
+    else if (rank - avail < n)
+        mine = base + (rank - avail);
+    else
+        *got = false;
+    if (rank == 0) {
+        const uint64_t used = min(static_cast<uint64_t>(k) - avail, n);
+        wq[0] = base + used;
+        wq[1] = base + n;
+    }
+    return mine;
+}
+
+template <bool APUSH>
+__global__ void __launch_bounds__(MAX_BLOCK) serve_kernel(ServeArgs a) {
+    extern __shared__ uint4 lds[];  // lds_layout(): dead frame, stack, wq, red (unused)
+    if (static_cast<uint32_t>(reinterpret_cast<uintptr_t>(lds)) != 0u) {
+        if (threadIdx.x == 0) atomicOr(a.error, DEV_ERR_LDS);
+        return;
+    }
+    const LdsLayout L = lds_layout(lds, static_cast<uint32_t>(a.stack_depth) + (APUSH ? 1u : 0u));
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        L.wq[wave][0] = 0;
+        L.wq[wave][1] = 0;
+    }
+    put_dead_frame(lds);
+    const uint32_t stride = blockDim.x * static_cast<uint32_t>(sizeof(uint4));
+    const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(L.stack + threadIdx.x));
+    const uint32_t chunk = static_cast<uint32_t>(a.chunk), sd = static_cast<uint32_t>(a.stack_depth);
+    uint32_t sp = base;
+    uint32_t c = 0, l = 0, r = 0, v = 0, cnt = 0, nd = 0;
+    uint32_t slot = 0, weight = 1;
+    uint64_t tsum = 0;
+    bool own = false;  // the lane holds a task it is counting
+    // The lane's run of finished tasks of one slot, added with one atomic each when the lane moves
+    // to another slot or finds nothing to do (so a request completes as soon as its last task).
+    uint32_t run_slot = 0;
+    uint64_t run_sum = 0, run_n = 0;
+    auto flush = [&]() {
+        if (!run_n) return;
+        atomicAdd(&a.count[run_slot], static_cast<unsigned long long>(run_sum));
+        const uint64_t before = __hip_atomic_fetch_add(&a.remaining[run_slot], 0ull - run_n, __ATOMIC_ACQ_REL,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+        if (before == run_n) {
+            // The request's last task: publish its count, then its sequence number.
+            const uint64_t total = __hip_atomic_load(&a.count[run_slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t q = __hip_atomic_load(&a.seq[run_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&a.done[run_slot].count, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&a.done[run_slot].seq, q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        run_sum = 0;
+        run_n = 0;
+    };
+    auto add_task = [&](uint32_t s, uint64_t x) {
+        if (run_n && s != run_slot) flush();
+        run_slot = s;
+        run_sum += x;
+        run_n += 1;
+    };
+    uint64_t idle_since = 0;
+    bool idle = false;
+
+    for (;;) {
+        if (v == 0) {
+            // Parked: account the finished task, then claim published tasks.
+            if (own) add_task(slot, tsum * weight);
+            own = false;
+            tsum = 0;
+            for (;;) {
+                bool got = false;
+                const uint64_t t = serve_claim(L.wq[wave], a, chunk, &got);
+                if (!got) break;
+                const uint4 w = load_ring(a.ring + (t & a.ring_mask));
+                const uint32_t N = w.w & 31u;
+                const uint32_t s = w.w >> 6;
+                weight = 1u + ((w.w >> 5) & 1u);
+                const uint32_t last = (1u << N) - 1u;
+                const uint32_t v0 = last & ~(w.x | w.y | w.z);
+                const uint32_t d0 = __popc(w.x);
+                if (d0 + 1u >= N || v0 == 0) {
+                    const uint64_t direct =
+                        d0 >= N ? (w.x == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
+                    add_task(s, direct * weight);
+                    continue;
+                }
+                if (N - 1u - d0 > sd) {
+                    // More frames than the server sized: flagged, counted as done (with 0).
+                    atomicOr(a.error, DEV_ERR_DEPTH);
+                    add_task(s, 0);
+                    continue;
+                }
+                c = w.x | ~last;
+                l = w.y;
+                r = w.z;
+                v = v0;
+                sp = base;
+                slot = s;
+                own = true;
+                break;
+            }
+            if (!own) flush();
+        }
+        const uint64_t live = __ballot(v != 0);
+        if (live == 0) {
+            // Nothing to run: wait for published tasks, the stop flag or the idle timeout.
+            const uint64_t now = wall_clock64();
+            if (!idle) {
+                idle = true;
+                idle_since = now;
+            }
+            const bool stop = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                                  &a.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) != 0;
+            const uint32_t exited = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&a.ctl->exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+            const bool retiring = static_cast<int>(exited - a.epoch) >= 0;
+            if (stop || retiring || now - idle_since > a.idle_ticks) break;
+            __builtin_amdgcn_s_sleep(64);
+            continue;
+        }
+        idle = false;
+        uint32_t wcnt = 0;  // unused: tasks of different requests are counted per lane
+        hot_loop<false, APUSH, false>(c, l, r, v, sp, cnt, wcnt, nd, live, stride,
+                                      live == __ballot(1) ? 0u : SERVE_IDLE_TRIPS);
+        tsum += cnt;
+        cnt = 0;
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        // Mark the instance retiring (a late wave of an older instance leaves a newer mark alone).
+        const uint32_t exited = __hip_atomic_load(&a.ctl->exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (static_cast<int>(a.epoch - exited) > 0)
+            __hip_atomic_store(&a.ctl->exited, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 __global__ void zero_u64_kernel(unsigned long long *p, uint64_t n) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
@@ -798,6 +1001,30 @@ hipError_t batch_fit(int block, int stack_depth, int waves_per_cu, int push, boo
     *apush = push == 2 || (push == 0 && prefer_always_push(occ[0], occ[1]));
     *blocks_per_cu = occ[*apush ? 1 : 0];
     return hipSuccess;
+}
+
+hipError_t serve_fit(int block, int stack_depth, int push, bool *apush, int *blocks_per_cu) {
+    int occ[2] = {0, 0};
+    for (int i = 0; i < 2; ++i) {
+        const void *k = i ? reinterpret_cast<const void *>(&serve_kernel<true>)
+                          : reinterpret_cast<const void *>(&serve_kernel<false>);
+        const size_t lds = dfs_lds_bytes(block, stack_depth + i);
+        hipError_t e = allow_lds(k, lds);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[i], k, block, lds);
+        if (e != hipSuccess) return e;
+    }
+    *apush = push == 2 || (push == 0 && prefer_always_push(occ[0], occ[1]));
+    *blocks_per_cu = occ[*apush ? 1 : 0];
+    return hipSuccess;
+}
+
+hipError_t launch_serve(const ServeArgs &a, hipStream_t stream) {
+    const size_t lds = dfs_lds_bytes(a.block, a.stack_depth + (a.apush ? 1 : 0));
+    if (a.apush)
+        serve_kernel<true><<<a.grid, a.block, lds, stream>>>(a);
+    else
+        serve_kernel<false><<<a.grid, a.block, lds, stream>>>(a);
+    return hipGetLastError();
 }
 
 hipError_t launch_batch(const BatchArgs &a, hipStream_t stream) {

@@ -57,6 +57,49 @@ struct BatchArgs {
     bool apush = false;  // unconditional push (stack_depth + 1 levels)
 };
 
+// Serving: a persistent kernel consuming a ring of batched tasks that the host keeps appending
+// to while it runs (nq_serve.cpp).  The host publishes tasks by raising `tail` in ServeCtl
+// (pinned host memory) after their copy into the ring has completed; waves claim published
+// tasks with a compare-and-swap on the device `head`, never beyond `tail`.  A task's tag
+// carries its request slot (tag >> 6); every finished task adds its count to count[slot] and
+// takes one off remaining[slot], and whoever takes the last one publishes {count, seq[slot]}
+// into done[slot] (pinned host memory).  Each wave exits after `idle_ticks` wall-clock ticks
+// without work, or at once when ServeCtl::stop is set and nothing is claimable; a wave that exits
+// stores its launch epoch in ServeCtl::exited.  From then on the instance retires: its waves
+// claim nothing new and exit once their tasks are counted, and the host launches a fresh
+// instance for the work that follows.  Every wait in the kernel is bounded by the idle timeout.
+struct ServeCtl {
+    unsigned long long tail;     // tasks published so far (written by the host)
+    unsigned long long claimed;  // head after a recent claim (device; a scheduling hint only)
+    unsigned int stop;           // exit once nothing is claimable (written by the host)
+    unsigned int exited;         // epoch of the last instance a wave of which has exited (device)
+};
+
+struct ServeDone {
+    unsigned long long count;   // the request's count once seq is stored
+    unsigned long long seq;     // the request's sequence number, stored last
+};
+
+struct ServeArgs {
+    const uint4 *ring;              // device: {cur, left, right, slot << 6 | twice << 5 | N}
+    unsigned long long ring_mask;   // ring capacity - 1 (a power of two)
+    unsigned long long *head;       // device: next unclaimed task index
+    ServeCtl *ctl;                  // pinned host memory
+    ServeDone *done;                // pinned host memory, one per slot
+    unsigned long long *remaining;  // device, one per slot: tasks not yet counted
+    unsigned long long *count;      // device, one per slot
+    const unsigned long long *seq;  // device, one per slot
+    unsigned long long *error;      // device: DEV_ERR_* bits
+    unsigned long long idle_ticks;  // wall-clock ticks a wave waits for work before it exits
+    unsigned int epoch;             // this launch's number (stored into ctl->exited on exit)
+    int block, grid, stack_depth, chunk;
+    bool apush;
+};
+
+// Enqueue one instance of the serving kernel.  serve_fit: push form and workgroups per CU.
+hipError_t launch_serve(const ServeArgs &a, hipStream_t stream);
+hipError_t serve_fit(int block, int stack_depth, int push, bool *apush, int *blocks_per_cu);
+
 // Expand a batched task list (uint4 {cur, left, right, tag}, N in tag bits 0..4) on the device,
 // up to `levels` rows, until no task has more than `target` free rows, keeping the order a
 // depth-first expansion of each task gives.  *tasks (hipMalloc'd) is replaced by a new buffer

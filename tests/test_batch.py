@@ -86,3 +86,28 @@ def test_batch_plan_replay_is_stable():
     c, st = plan.wait()
     plan.close()
     assert np.array_equal(c, eager)
+
+
+def _fixture(name):
+    import json
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "instance_counts.json")) as f:
+        fx = json.load(f)[name]
+    return fx
+
+
+@pytest.mark.gpu
+def test_batch_mix_every_instance_vs_oracle_fixture():
+    """The 512-instance mix (N up to 20): the batched launch, its graph replay and the per-instance
+    launches all equal the oracle's per-instance counts (tests/golden/instance_counts.json)."""
+    mix = workloads.batch_mix()
+    fx = _fixture("batch_mix")
+    assert fx["digest"] == workloads.instances_digest(mix)
+    counts, _ = nq.batch_count(mix)
+    assert [int(x) for x in counts] == fx["counts"]
+    plan = nq.BatchPlan(mix)
+    for _ in range(2):
+        again, _ = plan.run()
+        assert [int(x) for x in again] == fx["counts"]
+    plan.close()
+    per = [nq.instance_count(N, cols) for N, cols in mix]
+    assert per == fx["counts"]

@@ -40,6 +40,32 @@ def test_served_counts_equal_alone_counts():
 
 
 @pytest.mark.gpu
+def test_requests_join_the_running_kernel():
+    """Continuous batching into the persistent kernel: small requests submitted while a large one
+    is on the device are published into the same ring, counted by the running kernel and finish
+    long before the large one does; every count is exact."""
+    big = (19, [])                          # ~1e11 DFS nodes: about 0.1 s on the device
+    small = [(12, [0, 2]), (13, [1, 3]), (14, [0, 2, 4]), (12, [])]
+    with nq.Server() as srv:
+        t0 = time.perf_counter()
+        tb = srv.submit(*big)
+        time.sleep(0.005)                   # the large request is running now
+        ts = [(time.perf_counter(), srv.submit(N, cols)) for N, cols in small]
+        got_small = [(t_sub, srv.wait(t, timeout_ms=60000)) for t_sub, t in ts]
+        t_small_done = time.perf_counter()
+        cb, sb = srv.wait(tb, timeout_ms=120000)
+        t_big_done = t0 + sb["latency_ms"] / 1e3
+    assert cb == 4968057848
+    for (N, cols), (_, (c, _)) in zip(small, got_small):
+        want = 14200 if not cols else oracle.count_subtree(N, *oracle.encode_prefix(N, cols))
+        assert c == want, (N, cols)
+    # Every small request was submitted after the large one had started, and was done before it.
+    assert all(t_sub > t0 for t_sub, _ in got_small)
+    assert t_small_done < t_big_done, (t_small_done - t0, sb)
+    assert max(s["latency_ms"] for _, (_, s) in got_small) < sb["latency_ms"] / 2
+
+
+@pytest.mark.gpu
 def test_poisson_arrivals_overlap_and_are_exact():
     reqs = workloads.poisson_requests(80, lam=400.0, seed=5)
     alone = [nq.instance_count(N, cols) for _, N, cols in reqs]
@@ -62,6 +88,9 @@ def test_poisson_arrivals_overlap_and_are_exact():
     assert [c for c, _ in res] == alone
     assert all(s["latency_ms"] >= s["queue_ms"] >= 0 for _, s in res)
     assert st["batches"] >= 2
+    # Requests overlap on the device: some request was still in flight when a later one finished.
+    spans = [(at, at + s["latency_ms"] / 1e3) for (at, _, _), (_, s) in zip(reqs, res)]
+    assert any(a1 < a2 and e1 > e2 for a1, e1 in spans for a2, e2 in spans), "no two requests overlapped"
 
 
 @pytest.mark.gpu
@@ -76,12 +105,12 @@ def test_empty_board_requests(published):
 
 @pytest.mark.gpu
 def test_admission_budget_and_invalid_requests():
-    with nq.Server(max_pending_tasks=1000, slot_tasks=5000) as srv:
+    with nq.Server(max_pending_tasks=1000, slot_tasks=5000) as srv:   # a ring of 8192 tasks
         with pytest.raises(nq.NQError) as e:
             srv.submit(10, [0, 1])
         assert e.value.status == nq.NQ_ERR_ARG
         with pytest.raises(nq.NQError) as e:
-            srv.submit(19, [])          # expands to more tasks than one slot holds: never fits
+            srv.submit(19, [])          # expands to more tasks than the ring holds: never fits
         assert e.value.status == nq.NQ_ERR_NOMEM
         busy = 0
         tickets = []
@@ -93,3 +122,28 @@ def test_admission_budget_and_invalid_requests():
                 busy += 1
         counts = {srv.wait(t, timeout_ms=60000)[0] for t in tickets}
         assert counts == {oracle.count_subtree(15, *oracle.encode_prefix(15, [0, 2]))}
+
+
+@pytest.mark.gpu
+def test_served_poisson_set_vs_oracle_fixture():
+    """The 400 Poisson requests bench.py serves (N up to 19, whole boards included), submitted at
+    once: every served count equals the oracle's (tests/golden/instance_counts.json)."""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "instance_counts.json")) as f:
+        fx = json.load(f)["poisson"]
+    reqs = [(N, cols) for _, N, cols in workloads.poisson_requests(400, lam=1.0)]
+    assert fx["digest"] == workloads.instances_digest(reqs)
+    with nq.Server() as srv:
+        tickets = []
+        for N, cols in reqs:
+            for _ in range(60000):
+                try:
+                    tickets.append(srv.submit(N, cols))
+                    break
+                except nq.NQError as e:
+                    assert e.status == nq.NQ_ERR_BUSY
+                    time.sleep(0.001)
+            else:
+                raise AssertionError("request refused as busy for 60 s")
+        got = [srv.wait(t, timeout_ms=120000)[0] for t in tickets]
+    assert got == fx["counts"]
