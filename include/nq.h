@@ -192,11 +192,13 @@ int nq_batch_count(const nq_instance *inst, int n, const nq_config *cfg, uint64_
 int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *count);
 
 /* ---- serving with continuous batching (no reference counterpart) ---------------------
- * A server keeps one device-resident ring of tasks that a persistent kernel consumes while the
- * server keeps appending to it.  nq_server_submit validates a request, expands it into tasks on
+ * A server keeps one ring of tasks, in pinned host memory that the device reads directly, which a
+ * persistent kernel consumes while the server keeps appending to it (publishing is plain stores:
+ * no copy ever waits behind the running kernel).  nq_server_submit validates a request, expands it into tasks on
  * the caller's thread (until no task keeps more than depth_target free rows) and admits it
  * while the pending work fits max_pending_tasks (else NQ_ERR_BUSY, which a later retry may
- * pass; a request larger than the ring can never fit and is refused with NQ_ERR_NOMEM).  A
+ * pass; a request of more tasks than max_pending_tasks can never fit and is refused with
+ * NQ_ERR_NOMEM).  A
  * scheduler thread publishes requests into the ring in slices of at most batch_work estimated
  * DFS nodes, least estimated remaining work first (aged by waiting time), keeping no more than
  * two slices' work published and unclaimed, so that requests arriving while others run are
@@ -205,14 +207,16 @@ int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *c
  * request's count equals nq_instance_count's for the same instance.  Calls on one server are
  * thread-safe. */
 typedef struct nq_server_config {
-    int slots;                   /* streams for overlapping kernel instances (default 4)    */
-    uint64_t slot_tasks;         /* ring capacity in tasks (default 32M; rounded up to 2^k)  */
+    int waves_per_cu;            /* resident kernel waves per CU (default 0: all that fit) */
+    uint64_t slot_tasks;         /* ring capacity in tasks (default 8M = 128 MB pinned host  */
+                                 /* memory; rounded up to 2^k)                               */
     int slot_requests;           /* published slices not yet counted (default 4096)          */
     uint64_t max_pending_tasks;  /* admission budget of queued tasks (default 64M)         */
     int depth_target;            /* most free rows a task keeps (default 12): bounds both  */
                                  /* the LDS stack and one task's run time on its lane      */
     int block;                   /* workgroup size (default 128)                           */
-    int chunk;                   /* tasks per claim of a wave (default 16)                 */
+    int chunk;                   /* tickets a wave claims at once while published work is  */
+                                 /* plentiful, 1..64 (default 64); else one per idle lane  */
     double batch_work;           /* estimated DFS nodes per published slice (default 1e10) */
 } nq_server_config;
 

@@ -75,7 +75,7 @@ class _BatchStats(ctypes.Structure):
 
 
 class _ServerConfig(ctypes.Structure):
-    _fields_ = [("slots", ctypes.c_int), ("slot_tasks", ctypes.c_uint64), ("slot_requests", ctypes.c_int),
+    _fields_ = [("waves_per_cu", ctypes.c_int), ("slot_tasks", ctypes.c_uint64), ("slot_requests", ctypes.c_int),
                 ("max_pending_tasks", ctypes.c_uint64), ("depth_target", ctypes.c_int), ("block", ctypes.c_int),
                 ("chunk", ctypes.c_int), ("batch_work", ctypes.c_double)]
 

@@ -619,202 +619,182 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
 
 // ---- serving: persistent kernel over a ring the host appends to (ServeArgs in nq_kernels.h) ----
 
-// Trips an idle lane's wave keeps running its busy lanes before it looks for new tasks.
+// Trips a wave with waiting lanes keeps running its busy lanes before it looks for new tasks.
 constexpr uint32_t SERVE_IDLE_TRIPS = 64;
-// Compare-and-swap attempts per claim (a failed round leaves the lanes parked until the next one).
-constexpr int SERVE_CAS_TRIES = 64;
+// A wave with no task at all waits 1, 2, 4, ... 16 s_sleep(127) (about 3.4 us each at 2.4 GHz)
+// between looks at the host's tail, so that idle waves do not flood the bus with reads.
+constexpr uint32_t SERVE_IDLE_SLEEP_LOG2 = 4;
+// Published tickets beyond head that let a wave claim `chunk` tickets at once; with fewer, it
+// claims one per waiting lane, so that the last tasks spread over the waves.
+constexpr uint64_t SERVE_PLENTY = 64ull * SERVE_BUF;
 
-__device__ __forceinline__ uint64_t sys_load_u64(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+// The published tickets of this instance: tail[parity] - base (acquire: the entries below it
+// are written).  Every active lane reads the same word (one request per wave).
+__device__ __forceinline__ uint64_t serve_tail(const ServeArgs &a) {
+    const uint64_t t = __hip_atomic_load(&a.ctl->tail[a.parity], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    return first_lane_u64(t > a.base ? t - a.base : 0);
 }
 
-// A ring entry, read past the caches: the host's copy engine wrote it after the kernel started.
+// A ring entry in host memory: one 16-byte load past every cache (volatile: sc0 sc1).
 __device__ __forceinline__ uint4 load_ring(const uint4 *p) {
-    const unsigned int *w = reinterpret_cast<const unsigned int *>(p);
-    return make_uint4(__hip_atomic_load(w + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
-                      __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
-                      __hip_atomic_load(w + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
-                      __hip_atomic_load(w + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-}
-
-// Claim one published task index for every active lane (claim() with the global cursor bounded
-// by the host's tail): lanes take from the wave's LDS range first; the first active lane then
-// takes max(chunk, lanes left) more with a compare-and-swap on head, never past tail.  *got is
-// false for lanes that found nothing published (they stay parked).
-__device__ __forceinline__ uint64_t serve_claim(unsigned long long *wq_, const ServeArgs &a, uint32_t chunk, bool *got) {
-    volatile __attribute__((address_space(3))) unsigned long long *wq =
-        (volatile __attribute__((address_space(3))) unsigned long long *)wq_;
-    const uint64_t mask = __ballot(1);
-    const uint32_t k = __popcll(mask);
-    const uint32_t rank = lane_rank(mask);
-    const uint64_t next = wq[0];
-    const uint64_t end = wq[1];
-    const uint64_t avail = end - next;
-    if (avail >= k) {
-        if (rank == 0) wq[0] = next + k;
-        *got = true;
-        return next + rank;
-    }
-    uint64_t base = 0, n = 0;
-    // An instance retires once one of its waves has exited (ServeCtl::exited at or past its
-    // epoch): its waves then take no new tasks, finish the ones they hold and exit, so that a
-    // fresh instance with every wave takes the work from there.
-    const uint32_t exited = __builtin_amdgcn_readfirstlane(
-        __hip_atomic_load(&a.ctl->exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-    if (rank == 0 && static_cast<int>(exited - a.epoch) < 0) {
-        const uint64_t want = max(k - avail, static_cast<uint64_t>(chunk));
-        const uint64_t tail = sys_load_u64(&a.ctl->tail);
-        unsigned long long h = __hip_atomic_load(a.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int tries = 0; tries < SERVE_CAS_TRIES && h < tail; ++tries) {
-            const unsigned long long take = min(want, tail - h);
-            if (__hip_atomic_compare_exchange_strong(a.head, &h, h + take, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT)) {
-                base = h;
-                n = take;
-                // Progress for the host's scheduler (plain store: stores of two waves may land
-                // out of order, the host only uses it to size how much it publishes ahead).
-                __hip_atomic_store(&a.ctl->claimed, h + take, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                break;
-            }
-        }
-    }
-    base = first_lane_u64(base);
-    n = first_lane_u64(n);
-    uint64_t mine = 0;
-    *got = true;
-    if (rank < avail)
-        mine = next + rank;
-    else if (rank - avail < n)
-        mine = base + (rank - avail);
-    else
-        *got = false;
-    if (rank == 0) {
-        const uint64_t used = min(static_cast<uint64_t>(k) - avail, n);
-        wq[0] = base + used;
-        wq[1] = base + n;
-    }
-    return mine;
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    const v4u x = *reinterpret_cast<const volatile v4u *>(p);
+    return make_uint4(x.x, x.y, x.z, x.w);
 }
 
 template <bool APUSH>
 __global__ void __launch_bounds__(MAX_BLOCK) serve_kernel(ServeArgs a) {
-    extern __shared__ uint4 lds[];  // lds_layout(): dead frame, stack, wq, red (unused)
+    extern __shared__ uint4 lds[];  // lds_layout(), then SERVE_BUF entries per wave
     if (static_cast<uint32_t>(reinterpret_cast<uintptr_t>(lds)) != 0u) {
         if (threadIdx.x == 0) atomicOr(a.error, DEV_ERR_LDS);
         return;
     }
     const LdsLayout L = lds_layout(lds, static_cast<uint32_t>(a.stack_depth) + (APUSH ? 1u : 0u));
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63u) == 0) {
-        L.wq[wave][0] = 0;
-        L.wq[wave][1] = 0;
-    }
+    const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + wave;
+    uint4 *buf = reinterpret_cast<uint4 *>(L.red + MAX_WAVES_PER_BLOCK) + wave * SERVE_BUF;
     put_dead_frame(lds);
     const uint32_t stride = blockDim.x * static_cast<uint32_t>(sizeof(uint4));
     const uint32_t base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(L.stack + threadIdx.x));
-    const uint32_t chunk = static_cast<uint32_t>(a.chunk), sd = static_cast<uint32_t>(a.stack_depth);
+    const uint32_t sd = static_cast<uint32_t>(a.stack_depth);
+    // Wave-uniform claim state: tickets [bfirst, bend) claimed, [bfirst, bload) loaded into buf,
+    // [bfirst, bnext) handed to lanes; lim = published tickets as last read.
+    uint64_t bfirst = 0, bend = 0, bload = 0, bnext = 0, lim = 0;
     uint32_t sp = base;
     uint32_t c = 0, l = 0, r = 0, v = 0, cnt = 0, nd = 0;
-    uint32_t slot = 0, weight = 1;
+    uint32_t weight = 1;
     uint64_t tsum = 0;
     bool own = false;  // the lane holds a task it is counting
-    // The lane's run of finished tasks of one slot, added with one atomic each when the lane moves
-    // to another slot or finds nothing to do (so a request completes as soon as its last task).
+    // The lane's run of counted entries of one slot, added with one atomic each when the lane
+    // takes an entry of another slot or waits, so that a slice completes with its last task.
     uint32_t run_slot = 0;
     uint64_t run_sum = 0, run_n = 0;
     auto flush = [&]() {
         if (!run_n) return;
         atomicAdd(&a.count[run_slot], static_cast<unsigned long long>(run_sum));
-        const uint64_t before = __hip_atomic_fetch_add(&a.remaining[run_slot], 0ull - run_n, __ATOMIC_ACQ_REL,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-        if (before == run_n) {
-            // The request's last task: publish its count, then its sequence number.
-            const uint64_t total = __hip_atomic_load(&a.count[run_slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            const uint64_t q = __hip_atomic_load(&a.seq[run_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint64_t before = __hip_atomic_fetch_add(&a.finished[run_slot], static_cast<unsigned long long>(run_n),
+                                                       __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t want = __hip_atomic_load(&a.expect[run_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (before + run_n == want) {
+            // The slice's last entry: every other entry's count and the terminator's stores are
+            // visible (acquire above).  Reset the slot for its next slice (the host reuses it only
+            // after it has seen the record), then publish the count and, last, the sequence number.
+            const uint64_t total = __hip_atomic_load(&a.count[run_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t q = __hip_atomic_load(&a.seq[run_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.count[run_slot], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.finished[run_slot], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.expect[run_slot], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&a.done[run_slot].count, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&a.done[run_slot].seq, q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         run_sum = 0;
         run_n = 0;
     };
-    auto add_task = [&](uint32_t s, uint64_t x) {
-        if (run_n && s != run_slot) flush();
-        run_slot = s;
-        run_sum += x;
-        run_n += 1;
-    };
     uint64_t idle_since = 0;
-    bool idle = false;
+    uint32_t idle_polls = 0;
 
     for (;;) {
-        if (v == 0) {
-            // Parked: account the finished task, then claim published tasks.
-            if (own) add_task(slot, tsum * weight);
+        // (Every lane of the wave is active here.)  Account the tasks that have finished.
+        if (v == 0 && own) {
+            run_sum += tsum * weight;
+            run_n += 1;
             own = false;
             tsum = 0;
-            for (;;) {
-                bool got = false;
-                const uint64_t t = serve_claim(L.wq[wave], a, chunk, &got);
-                if (!got) break;
-                const uint4 w = load_ring(a.ring + (t & a.ring_mask));
+        }
+        uint64_t need = __ballot(v == 0);
+        while (need) {
+            if (bnext == bload) {
+                if (bload == bend) {
+                    // Claim: `chunk` tickets while published work is plentiful, else one per
+                    // waiting lane.
+                    const uint32_t k = __popcll(need);
+                    uint64_t first = 0, take = 0;
+                    if (lane == 0) {
+                        const uint64_t h = __hip_atomic_load(a.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        take = (lim > h && lim - h >= SERVE_PLENTY) ? max(k, static_cast<uint32_t>(a.chunk)) : k;
+                        first = __hip_atomic_fetch_add(a.head, static_cast<unsigned long long>(take), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                        const uint64_t rec = (static_cast<uint64_t>(a.epoch & 0xFFFFu) << SERVE_CLAIM_EPOCH_SHIFT) |
+                                             ((first + take) & ((1ull << SERVE_CLAIM_EPOCH_SHIFT) - 1));
+                        __hip_atomic_store(&a.claimed[gwave], rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    }
+                    first = first_lane_u64(first);
+                    take = first_lane_u64(take);
+                    bfirst = bload = bnext = first;
+                    bend = first + take;
+                }
+                if (bload >= lim) lim = serve_tail(a);
+                const uint64_t n = (lim > bload ? min(bend, lim) : bload) - bload;  // published, not loaded
+                if (n == 0) break;
+                if (lane < n) buf[bload - bfirst + lane] = load_ring(a.ring + ((a.base + bload + lane) & a.ring_mask));
+                bload += n;
+            }
+            // Hand the loaded entries to the waiting lanes, lowest rank first.
+            const uint64_t avail = bload - bnext;
+            const uint32_t rank = lane_rank(need);
+            if (((need >> lane) & 1u) && rank < avail) {
+                const uint4 w = buf[bnext - bfirst + rank];
+                const uint32_t s = w.w >> SERVE_TAG_SLOT_SHIFT;
+                if (run_n && s != run_slot) flush();
+                run_slot = s;
                 const uint32_t N = w.w & 31u;
-                const uint32_t s = w.w >> 6;
                 weight = 1u + ((w.w >> 5) & 1u);
                 const uint32_t last = (1u << N) - 1u;
                 const uint32_t v0 = last & ~(w.x | w.y | w.z);
                 const uint32_t d0 = __popc(w.x);
-                if (d0 + 1u >= N || v0 == 0) {
+                if (w.w & SERVE_TAG_TERM) {
+                    // The slice's terminator: its entry count and sequence number, stored before
+                    // this lane's own count (the flush's release orders them).
+                    __hip_atomic_store(&a.seq[s], (static_cast<uint64_t>(w.z) << 32) | w.y, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&a.expect[s], static_cast<unsigned long long>(w.x), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    run_n += 1;
+                } else if (d0 + 1u >= N || v0 == 0) {
                     const uint64_t direct =
                         d0 >= N ? (w.x == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
-                    add_task(s, direct * weight);
-                    continue;
-                }
-                if (N - 1u - d0 > sd) {
+                    run_sum += direct * weight;
+                    run_n += 1;
+                } else if (N - 1u - d0 > sd) {
                     // More frames than the server sized: flagged, counted as done (with 0).
                     atomicOr(a.error, DEV_ERR_DEPTH);
-                    add_task(s, 0);
-                    continue;
+                    run_n += 1;
+                } else {
+                    c = w.x | ~last;
+                    l = w.y;
+                    r = w.z;
+                    v = v0;
+                    sp = base;
+                    own = true;
                 }
-                c = w.x | ~last;
-                l = w.y;
-                r = w.z;
-                v = v0;
-                sp = base;
-                slot = s;
-                own = true;
-                break;
             }
-            if (!own) flush();
+            bnext += min(avail, static_cast<uint64_t>(__popcll(need)));
+            need = __ballot(v == 0);
         }
+        // Lanes left waiting hand in their counts, so that finished slices complete now.
+        if (v == 0) flush();
         const uint64_t live = __ballot(v != 0);
         if (live == 0) {
-            // Nothing to run: wait for published tasks, the stop flag or the idle timeout.
+            // Nothing to run: every lane waits for its ticket to be published.  Leave when the
+            // host has asked this instance to stop (it does so only with every published task
+            // counted, and publishes no more for it) or after orphan_ticks.
+            const uint32_t stop_ep = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&a.ctl->stop_epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+            if (stop_ep == a.epoch) break;
             const uint64_t now = wall_clock64();
-            if (!idle) {
-                idle = true;
-                idle_since = now;
-            }
-            const bool stop = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
-                                  &a.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) != 0;
-            const uint32_t exited = __builtin_amdgcn_readfirstlane(
-                __hip_atomic_load(&a.ctl->exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-            const bool retiring = static_cast<int>(exited - a.epoch) >= 0;
-            if (stop || retiring || now - idle_since > a.idle_ticks) break;
-            __builtin_amdgcn_s_sleep(64);
+            if (idle_polls == 0) idle_since = now;
+            if (now - idle_since > a.orphan_ticks) break;
+            for (uint32_t i = 0; i < (1u << min(idle_polls, SERVE_IDLE_SLEEP_LOG2)); ++i) __builtin_amdgcn_s_sleep(127);
+            ++idle_polls;
             continue;
         }
-        idle = false;
+        idle_polls = 0;
         uint32_t wcnt = 0;  // unused: tasks of different requests are counted per lane
         hot_loop<false, APUSH, false>(c, l, r, v, sp, cnt, wcnt, nd, live, stride,
                                       live == __ballot(1) ? 0u : SERVE_IDLE_TRIPS);
         tsum += cnt;
         cnt = 0;
-    }
-    if ((threadIdx.x & 63u) == 0) {
-        // Mark the instance retiring (a late wave of an older instance leaves a newer mark alone).
-        const uint32_t exited = __hip_atomic_load(&a.ctl->exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        if (static_cast<int>(a.epoch - exited) > 0)
-            __hip_atomic_store(&a.ctl->exited, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1003,12 +983,17 @@ hipError_t batch_fit(int block, int stack_depth, int waves_per_cu, int push, boo
     return hipSuccess;
 }
 
+size_t serve_lds_bytes(int block, int levels) {
+    // The counting layout plus SERVE_BUF ring entries per wave.
+    return dfs_lds_bytes(block, levels) + static_cast<size_t>(block / 64) * SERVE_BUF * sizeof(uint4);
+}
+
 hipError_t serve_fit(int block, int stack_depth, int push, bool *apush, int *blocks_per_cu) {
     int occ[2] = {0, 0};
     for (int i = 0; i < 2; ++i) {
         const void *k = i ? reinterpret_cast<const void *>(&serve_kernel<true>)
                           : reinterpret_cast<const void *>(&serve_kernel<false>);
-        const size_t lds = dfs_lds_bytes(block, stack_depth + i);
+        const size_t lds = serve_lds_bytes(block, stack_depth + i);
         hipError_t e = allow_lds(k, lds);
         if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[i], k, block, lds);
         if (e != hipSuccess) return e;
@@ -1019,7 +1004,7 @@ hipError_t serve_fit(int block, int stack_depth, int push, bool *apush, int *blo
 }
 
 hipError_t launch_serve(const ServeArgs &a, hipStream_t stream) {
-    const size_t lds = dfs_lds_bytes(a.block, a.stack_depth + (a.apush ? 1 : 0));
+    const size_t lds = serve_lds_bytes(a.block, a.stack_depth + (a.apush ? 1 : 0));
     if (a.apush)
         serve_kernel<true><<<a.grid, a.block, lds, stream>>>(a);
     else

@@ -58,43 +58,71 @@ struct BatchArgs {
 };
 
 // Serving: a persistent kernel consuming a ring of batched tasks that the host keeps appending
-// to while it runs (nq_serve.cpp).  The host publishes tasks by raising `tail` in ServeCtl
-// (pinned host memory) after their copy into the ring has completed; waves claim published
-// tasks with a compare-and-swap on the device `head`, never beyond `tail`.  A task's tag
-// carries its request slot (tag >> 6); every finished task adds its count to count[slot] and
-// takes one off remaining[slot], and whoever takes the last one publishes {count, seq[slot]}
-// into done[slot] (pinned host memory).  Each wave exits after `idle_ticks` wall-clock ticks
-// without work, or at once when ServeCtl::stop is set and nothing is claimable; a wave that exits
-// stores its launch epoch in ServeCtl::exited.  From then on the instance retires: its waves
-// claim nothing new and exit once their tasks are counted, and the host launches a fresh
-// instance for the work that follows.  Every wait in the kernel is bounded by the idle timeout.
+// to while it runs (nq_serve.cpp).  Everything the host writes while the kernel runs lives in
+// pinned, coherent host memory that the kernel reads directly (a copy would be a kernel waiting
+// for a CU behind the persistent one): the ring, ServeCtl, and the completion records the
+// kernel writes back.
+//
+// Ring entries are 16 bytes: {cur, left, right, tag}, tag = slot << 7 | TERM | twice << 5 | N.
+// A published slice is its tasks followed by one terminator entry (tag bit SERVE_TAG_TERM,
+// x = tasks + 1, y/z = the slice's sequence number, low/high).
+//
+// Claims are tickets: a wave takes tickets with one fetch-and-add on the device `head` (a claim
+// never fails or retries, however many waves claim at once) and loads their entries with one
+// coalesced read into its LDS buffer; ticket t is ring position base + t, and is published once
+// base + t < tail[parity].  A wave whose tickets are not yet published waits for the host to
+// raise the tail.  Each wave records the end of its latest claim in its `claimed` entry (epoch in
+// bits 48..63, end - base below), which the host reads as the claim cursor.
+//
+// Every finished task adds its count to count[slot] and one to finished[slot]; the lane that
+// takes a terminator stores its total into expect[slot] and its sequence number into seq[slot]
+// (device memory) before it adds its own one.  Whoever brings finished[slot] to expect[slot]
+// resets count, finished and expect to zero and writes {count, seq} into done[slot].
+//
+// The host stops an instance only with every published task counted: it stores the instance's
+// epoch into stop_epoch and from then on publishes into the other tail, so the stopped instance
+// sees no new task; its waves exit once idle.  The next instance starts on the same stream
+// (after the stopped one has exited) with its head zeroed, the other parity and base = the tail
+// at the stop.  A wave without work for `orphan_ticks` wall-clock ticks also exits, so that an
+// instance whose host has gone away ends by itself.
+constexpr uint32_t SERVE_TAG_TERM = 1u << 6;
+constexpr int SERVE_TAG_SLOT_SHIFT = 7;
+constexpr int SERVE_CLAIM_EPOCH_SHIFT = 48;  // claimed[] entries: epoch << 48 | (end - base)
+constexpr int SERVE_BUF = 64;                // tickets a wave claims and loads at most at once
+
 struct ServeCtl {
-    unsigned long long tail;     // tasks published so far (written by the host)
-    unsigned long long claimed;  // head after a recent claim (device; a scheduling hint only)
-    unsigned int stop;           // exit once nothing is claimable (written by the host)
-    unsigned int exited;         // epoch of the last instance a wave of which has exited (device)
+    unsigned long long tail[2];  // tasks published so far, per instance parity (host)
+    unsigned int stop_epoch;     // the instance asked to stop (host)
+    unsigned int pad;
 };
 
 struct ServeDone {
-    unsigned long long count;   // the request's count once seq is stored
-    unsigned long long seq;     // the request's sequence number, stored last
+    unsigned long long count;  // the slice's count once seq is stored
+    unsigned long long seq;    // the slice's sequence number, stored last
 };
 
 struct ServeArgs {
-    const uint4 *ring;              // device: {cur, left, right, slot << 6 | twice << 5 | N}
+    const uint4 *ring;              // host
     unsigned long long ring_mask;   // ring capacity - 1 (a power of two)
-    unsigned long long *head;       // device: next unclaimed task index
-    ServeCtl *ctl;                  // pinned host memory
-    ServeDone *done;                // pinned host memory, one per slot
-    unsigned long long *remaining;  // device, one per slot: tasks not yet counted
-    unsigned long long *count;      // device, one per slot
-    const unsigned long long *seq;  // device, one per slot
+    unsigned long long base;        // ring position of ticket 0
+    unsigned long long *head;       // device: next ticket (zero at launch)
+    const ServeCtl *ctl;            // host
+    ServeDone *done;                // host, one per slot
+    unsigned long long *claimed;    // host, one per wave: epoch and end of its latest claim
+    unsigned long long *finished;   // device, one per slot: entries counted (zero between slices)
+    unsigned long long *count;      // device, one per slot (zero between slices)
+    unsigned long long *expect;     // device, one per slot: entries of the slice (0: not known yet)
+    unsigned long long *seq;        // device, one per slot: the slice's sequence number
     unsigned long long *error;      // device: DEV_ERR_* bits
-    unsigned long long idle_ticks;  // wall-clock ticks a wave waits for work before it exits
-    unsigned int epoch;             // this launch's number (stored into ctl->exited on exit)
-    int block, grid, stack_depth, chunk;
+    unsigned long long orphan_ticks;  // wall-clock ticks a wave waits for work before it exits
+    unsigned int epoch;             // this launch's number
+    int parity;                     // which tail this instance reads
+    int chunk;                      // tickets a wave claims while published work is plentiful
+    int block, grid, stack_depth;
     bool apush;
 };
+
+size_t serve_lds_bytes(int block, int levels);
 
 // Enqueue one instance of the serving kernel.  serve_fit: push form and workgroups per CU.
 hipError_t launch_serve(const ServeArgs &a, hipStream_t stream);

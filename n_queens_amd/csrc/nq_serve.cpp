@@ -9,25 +9,33 @@
 //   * it publishes requests in slices (at most batch_work estimated DFS nodes each), least
 //     estimated remaining work first, aged by waiting time, and only while the published but
 //     unclaimed work stays under a lookahead of two slices, so a request that arrives behind a
-//     large one waits for at most that much work (the claim cursor is read from the device when
-//     the kernel's progress hint would hold publication back);
-//   * a slice is packed into pinned staging and copied into the ring on a copy stream outside
-//     the server lock; once the copy has completed the scheduler raises the ring's tail in
-//     pinned host memory, which the running kernel reads;
+//     large one waits for at most that much work (every kernel wave records the end of its last
+//     claim in host memory; the largest record is the claim cursor);
+//   * the ring, the tail and the completion records live in pinned, coherent host memory that
+//     the kernel reads and writes directly: publishing a slice is plain stores (its tasks and a
+//     terminator entry carrying its size and sequence number, outside the server lock) followed
+//     by raising the tail, and no copy is ever queued behind the running kernel;
 //   * every published slice occupies one of slot_requests slots; the kernel adds every finished
-//     task's count to its slot and, when the slice's last task is counted, writes {count,
-//     sequence number} into pinned host memory, where the scheduler finds it; the slice's ring
-//     space is then free, and a request is done when its last slice is counted;
-//   * kernel waves exit after a short idle time without work (every wait in the kernel is
-//     bounded), and an instance one of whose waves has exited retires: it claims nothing new and
-//     ends when its tasks are counted.  The scheduler launches a new instance, on the next of
-//     `slots` streams, whenever published tasks are unclaimed and no instance is running or the
-//     latest one retires.  Instances share the ring's claim cursor, so they may overlap.
+//     task's count to its slot and, when the slice's last entry is counted, writes {count,
+//     sequence number} into host memory, where the scheduler finds it; the slice's ring space is
+//     then free, and a request is done when its last slice is counted;
+//   * kernel waves claim tasks as tickets with a fetch-and-add on the device cursor (a claim
+//     never fails, however many waves claim at once) and read their entries with one coalesced
+//     load per claim; a ticket past the published tail waits for the host to publish it;
+//   * one kernel instance runs while requests are in flight; the scheduler stops it once every
+//     published task has been counted and nothing has arrived for IDLE_STOP_MS (so that an idle
+//     server leaves the device to other work), and launches the next one on the same stream when
+//     work comes again.
 //
 // A request's count (the sum over its tasks) is exactly the one nq_instance_count gives.
+//
+// With NQ_SERVE_TRACE=<file> in the environment the scheduler records its events (slices
+// published, kernel instances launched, instances seen retiring, slices counted) and writes them
+// to <file> at close, one per line: "<ms since open> <event> <fields>".
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -37,6 +45,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -81,16 +90,25 @@ double est_task_nodes(int m) {
 // ring space is freed as soon as it is counted, whatever the rest of its request does).
 struct Slice {
     std::shared_ptr<Request> req;
-    uint64_t begin = 0, end = 0;
+    uint64_t begin = 0, end = 0;  // ring positions: the tasks, then the terminator at end - 1
+    uint64_t ntasks = 0;
     double work = 0;
     int slot = -1;
     uint64_t seq = 0;
     bool counted = false;
 };
 
-constexpr uint64_t STAGING_TASKS = 1u << 20;  // tasks per ring copy
-constexpr double IDLE_US = 200;               // a kernel wave's idle time before it exits
+constexpr uint64_t SLICE_MAX_TASKS = 1u << 20;  // tasks one slice writes into the ring at most
+constexpr double IDLE_STOP_MS = 0.2;          // an idle server stops its kernel after this long
+constexpr double ORPHAN_S = 30;               // a kernel wave without work for this long exits
 constexpr double CLOSE_DRAIN_MS = 120000;     // close() waits this long for requests in flight
+
+struct TraceEvent {
+    double t;
+    char kind;  // 'P' slice published, 'L' instance launched, 'S' instance stopped, 'C' slice counted
+    uint64_t a, b, c;
+    double d;
+};
 
 }  // namespace
 
@@ -98,23 +116,26 @@ struct nq_server {
     nq_server_config cfg{};
     int device = 0, cus = 0, wall_khz = 0;
     // Device memory.
+    unsigned long long *head = nullptr, *finished = nullptr, *count = nullptr, *expect = nullptr, *seq = nullptr,
+                       *error = nullptr;
+    // Pinned, coherent host memory.
     uint4 *ring = nullptr;
     uint64_t ring_cap = 0;  // a power of two
-    unsigned long long *head = nullptr, *remaining = nullptr, *count = nullptr, *seq = nullptr, *error = nullptr;
-    // Pinned host memory.
     ServeCtl *ctl = nullptr;
     ServeDone *done = nullptr;
-    uint4 *staging = nullptr;
-    unsigned long long *slot_init = nullptr;  // [remaining, count, seq] for a slot being opened
-    hipStream_t copy_stream = nullptr;
-    hipEvent_t copy_ev = nullptr;
-    std::vector<hipStream_t> kstreams;
-    std::vector<hipEvent_t> kevents;
-    std::vector<bool> kused;
+    unsigned long long *claimed = nullptr;  // one entry per wave of an instance
+    uint64_t waves = 0;                     // waves of one kernel instance
+    hipStream_t stream = nullptr;
+    hipEvent_t kev = nullptr;  // recorded after the latest instance
     int block = 0, grid = 0, stack_depth = 0;
     bool apush = false;
-    unsigned epoch = 0;   // number of the latest kernel instance
-    int next_stream = 0;  // stream of the next instance
+    unsigned epoch = 0;      // number of the latest kernel instance
+    bool running = false;    // an instance was launched and has not been seen to end
+    bool stopping = false;   // ... and it was asked to stop
+    uint64_t base = 0;       // ring position of the latest instance's ticket 0
+    int parity = 0;          // the tail the next (or running) instance reads
+    uint64_t gen_base = 0;   // ring position of the next instance's ticket 0 (the tail at the last stop)
+    double idle_since = -1;  // since when nothing is pending or in flight (ms)
     // Scheduling state, under mu.
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -123,36 +144,32 @@ struct nq_server {
     std::vector<Slice *> slot_slice;  // slice counted in each slot (null: free)
     std::vector<int> free_slots;
     std::deque<Slice> published;  // ring FIFO, freed from the front as slices are counted
-    uint64_t tail = 0;            // tasks published (host copy of ctl->tail)
+    uint64_t tail = 0;            // ring entries published
     uint64_t next_ticket = 1, next_seq = 1, pending_tasks = 0;  // next_seq numbers slices
     int inflight = 0;  // slices published and not yet counted
-    double head_checked = -1e30;  // last read of the device claim cursor for a launch (ms)
-    double head_read = -1e30;     // last read of it for the lookahead (ms)
-    uint64_t claimed_seen = 0;    // largest claim cursor value seen (a lower bound of it)
+    double head_read = -1e30;   // last scan of the waves' claim records (ms)
+    uint64_t claimed_seen = 0;  // largest claim record seen (a lower bound of the claim cursor)
     bool stop = false;
     double t_stop = 0;  // when close() was called
     int error_code = NQ_OK;
     std::string error_msg;
     std::thread thr;
-    // Statistics.
+    // Statistics and the event trace (NQ_SERVE_TRACE).
     uint64_t completed = 0, slices = 0, solutions = 0, launches = 0;
+    std::string trace_path;
+    std::vector<TraceEvent> trace;
     double busy_ms = 0, busy_since = -1, t_open = 0;
 };
 
 static void free_server(nq_server *S) {
-    if (S->ring) (void)hipFree(S->ring);
-    for (unsigned long long *p : {S->head, S->remaining, S->count, S->seq, S->error})
+    for (unsigned long long *p : {S->head, S->finished, S->count, S->expect, S->seq, S->error})
         if (p) (void)hipFree(p);
+    if (S->ring) (void)hipHostFree(S->ring);
     if (S->ctl) (void)hipHostFree(S->ctl);
     if (S->done) (void)hipHostFree(S->done);
-    if (S->staging) (void)hipHostFree(S->staging);
-    if (S->slot_init) (void)hipHostFree(S->slot_init);
-    if (S->copy_ev) (void)hipEventDestroy(S->copy_ev);
-    if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
-    for (hipEvent_t e : S->kevents)
-        if (e) (void)hipEventDestroy(e);
-    for (hipStream_t s : S->kstreams)
-        if (s) (void)hipStreamDestroy(s);
+    if (S->claimed) (void)hipHostFree(S->claimed);
+    if (S->kev) (void)hipEventDestroy(S->kev);
+    if (S->stream) (void)hipStreamDestroy(S->stream);
     delete S;
 }
 
@@ -175,13 +192,13 @@ static double unclaimed_work(const nq_server *S, uint64_t claimed) {
 }
 
 // The next slice to publish (lock held): the pending request with the least estimated remaining
-// work, aged by its waiting time; its tasks up to the slice budget and the ring and staging
-// space.  Returns false when nothing can be published now (no request, no free slot, no space).
+// work, aged by its waiting time; its tasks up to the slice budget, the ring space and
+// SLICE_MAX_TASKS.  Returns false when nothing can be published now (no request, no free slot, no space).
 static bool pick_slice(nq_server *S, double now, Slice *out) {
     if (S->pending.empty() || S->free_slots.empty()) return false;
     const uint64_t front = S->published.empty() ? S->tail : S->published.front().begin;
     const uint64_t space = S->ring_cap - (S->tail - front);
-    if (space == 0) return false;
+    if (space < 2) return false;  // a task and the terminator
     auto key = [&](const std::shared_ptr<Request> &r) {
         const double rem = static_cast<double>(r->tasks.size() - r->next) * r->task_work;
         return rem / 1e9 - 2.0 * (now - r->t_submit);  // ~1 ms per 1e9 nodes; 2x aging
@@ -191,102 +208,118 @@ static bool pick_slice(nq_server *S, double now, Slice *out) {
         if (!best || key(r) < key(best)) best = r;
     const uint64_t left = best->tasks.size() - best->next;
     uint64_t take = static_cast<uint64_t>(std::max(1.0, S->cfg.batch_work / best->task_work));
-    take = std::min<uint64_t>({take, left, space, STAGING_TASKS});
+    take = std::min<uint64_t>({take, left, space - 1, SLICE_MAX_TASKS});
     out->req = best;
     out->begin = S->tail;
-    out->end = S->tail + take;
+    out->end = S->tail + take + 1;
+    out->ntasks = take;
     out->work = static_cast<double>(take) * best->task_work;
     return true;
 }
 
-// Copy a slice into the ring with its slot's initial state (tasks left = the slice's tasks,
-// count 0, the slice's sequence number) and wait for the copies; called without the lock.
-static hipError_t copy_slice(nq_server *S, const Slice &sl, size_t first) {
+// Write a slice's tasks (tagged with its slot) and its terminator into its ring positions; the
+// scheduler publishes them by raising the tail afterwards.  Called without the lock: the
+// positions and the slot are the slice's alone until then.
+static void write_slice(nq_server *S, const Slice &sl, size_t first) {
     const Request &r = *sl.req;
-    const uint64_t n = sl.end - sl.begin;
-    for (uint64_t j = 0; j < n; ++j) {
+    const uint64_t mask = S->ring_cap - 1;
+    const uint32_t tag = static_cast<uint32_t>(sl.slot) << SERVE_TAG_SLOT_SHIFT;
+    for (uint64_t j = 0; j < sl.ntasks; ++j) {
         uint4 u = r.tasks[first + j];
-        u.w = (static_cast<uint32_t>(sl.slot) << 6) | (u.w & 63u);
-        S->staging[j] = u;
+        u.w = tag | (u.w & 63u);
+        S->ring[(sl.begin + j) & mask] = u;
     }
-    S->slot_init[0] = n;
-    S->slot_init[1] = 0;
-    S->slot_init[2] = sl.seq;
-    hipError_t e = hipMemcpyAsync(S->remaining + sl.slot, &S->slot_init[0], 8, hipMemcpyHostToDevice, S->copy_stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(S->count + sl.slot, &S->slot_init[1], 8, hipMemcpyHostToDevice, S->copy_stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(S->seq + sl.slot, &S->slot_init[2], 8, hipMemcpyHostToDevice, S->copy_stream);
-    // Ring positions wrap: up to two contiguous pieces.
-    const uint64_t pos = sl.begin & (S->ring_cap - 1);
-    const uint64_t n1 = std::min(n, S->ring_cap - pos);
-    if (e == hipSuccess && n1)
-        e = hipMemcpyAsync(S->ring + pos, S->staging, n1 * sizeof(uint4), hipMemcpyHostToDevice, S->copy_stream);
-    if (e == hipSuccess && n > n1)
-        e = hipMemcpyAsync(S->ring, S->staging + n1, (n - n1) * sizeof(uint4), hipMemcpyHostToDevice,
-                           S->copy_stream);
-    if (e == hipSuccess) e = hipEventRecord(S->copy_ev, S->copy_stream);
-    if (e == hipSuccess) e = hipEventSynchronize(S->copy_ev);
-    return e;
+    S->ring[(sl.end - 1) & mask] = make_uint4(static_cast<uint32_t>(sl.ntasks + 1), static_cast<uint32_t>(sl.seq),
+                                              static_cast<uint32_t>(sl.seq >> 32), tag | SERVE_TAG_TERM);
 }
 
-// Launch a kernel instance when published tasks are unclaimed and no instance is running, or
-// the latest one has begun to exit (its idle waves leave; busy ones finish their tasks first).
-// With an instance running the device's claim cursor decides, read at most every 100 us (lock
-// held; the read waits for a small copy).  Returns the launch error, if any.
-static hipError_t maybe_launch(nq_server *S, double now) {
-    if (S->inflight == 0) return hipSuccess;
-    int running = 0;
-    for (size_t i = 0; i < S->kstreams.size(); ++i) {
-        if (!S->kused[i]) continue;
-        const hipError_t q = hipEventQuery(S->kevents[i]);
-        if (q == hipErrorNotReady) {
-            ++running;
-        } else {
-            S->kused[i] = false;
-            if (q != hipSuccess) return q;
+// The largest claim end the latest instance's waves have recorded: a lower bound of its claim
+// cursor (tickets past the tail included), equal to it once the waves' stores have landed.
+static uint64_t claimed_max(const nq_server *S) {
+    const uint64_t tag = static_cast<uint64_t>(S->epoch & 0xFFFFu);
+    const uint64_t low = (1ull << SERVE_CLAIM_EPOCH_SHIFT) - 1;
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < S->waves; ++i) {
+        const uint64_t rec = __atomic_load_n(&S->claimed[i], __ATOMIC_RELAXED);
+        if ((rec >> SERVE_CLAIM_EPOCH_SHIFT) == tag) m = std::max(m, rec & low);
+    }
+    return S->base + m;
+}
+
+// Ask the running instance to stop and switch publication to the other tail, which the next
+// instance reads from the current tail on (lock held).  Called with every published task counted
+// (or, at close after an error, with the instance left to count what it has): the stopped
+// instance sees no task published from now on, and its waves exit once idle.
+static void stop_instance(nq_server *S) {
+    __atomic_store_n(&S->ctl->stop_epoch, S->epoch, __ATOMIC_RELEASE);
+    S->stopping = true;
+    S->parity ^= 1;
+    S->gen_base = S->tail;
+    __atomic_store_n(&S->ctl->tail[S->parity], S->tail, __ATOMIC_RELEASE);
+    if (!S->trace_path.empty()) S->trace.push_back({now_ms() - S->t_open, 'S', S->epoch, S->tail, 0, 0});
+}
+
+// Keep one kernel instance running while slices are in flight (lock held): note an instance
+// that has ended, stop the running one once the server has been idle for IDLE_STOP_MS (every
+// published task counted, nothing pending), and launch a new one when slices are in flight and
+// none runs (or the running one is stopping: the new one waits for it on the stream).
+static hipError_t manage_instance(nq_server *S, double now) {
+    if (S->running) {
+        const hipError_t q = hipEventQuery(S->kev);
+        if (q == hipSuccess) {
+            S->running = false;
+            if (!S->stopping && S->inflight > 0) {
+                set_error(S, NQ_ERR_STATE, "serving kernel exited with requests in flight");
+                return hipSuccess;
+            }
+        } else if (q != hipErrorNotReady) {
+            return q;
         }
     }
-    if (running > 0) {
-        const unsigned exited = __atomic_load_n(&S->ctl->exited, __ATOMIC_ACQUIRE);
-        if (exited != S->epoch || now - S->head_checked < 0.1) return hipSuccess;
-        S->head_checked = now;
-        unsigned long long h = 0;
-        hipError_t e = hipMemcpyAsync(&h, S->head, sizeof h, hipMemcpyDeviceToHost, S->copy_stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(S->copy_stream);
-        if (e != hipSuccess) return e;
-        S->claimed_seen = std::max<uint64_t>(S->claimed_seen, h);
-        if (h >= S->tail) return hipSuccess;  // every published task is claimed: busy lanes finish them
+    if (S->inflight == 0 && S->pending.empty()) {
+        if (S->idle_since < 0) S->idle_since = now;
+        if (S->running && !S->stopping && (now - S->idle_since > IDLE_STOP_MS || S->stop)) {
+            stop_instance(S);
+        }
+        return hipSuccess;
     }
-    // (No instance running while requests are in flight: their tasks are unclaimed, since waves
-    // exit only with every claimed task counted and collect() ran first.)
-    const int i = S->next_stream;
-    if (S->kused[i]) return hipSuccess;  // every stream still runs an instance
-    S->next_stream = (i + 1) % static_cast<int>(S->kstreams.size());
+    S->idle_since = -1;
+    if (S->inflight == 0 || (S->running && !S->stopping)) return hipSuccess;
+    // The next instance: ticket 0 is the first task published since the last one stopped (or
+    // since open), and its cursor is zeroed behind the stopping instance on the same stream.
     ServeArgs a{};
     a.ring = S->ring;
     a.ring_mask = S->ring_cap - 1;
+    a.base = S->gen_base;
+    a.parity = S->parity;
     a.head = S->head;
     a.ctl = S->ctl;
     a.done = S->done;
-    a.remaining = S->remaining;
+    a.claimed = S->claimed;
+    a.finished = S->finished;
     a.count = S->count;
+    a.expect = S->expect;
     a.seq = S->seq;
     a.error = S->error;
-    a.idle_ticks = static_cast<unsigned long long>(IDLE_US * S->wall_khz / 1000.0);
+    a.orphan_ticks = static_cast<unsigned long long>(ORPHAN_S * 1000.0 * S->wall_khz);
     a.epoch = ++S->epoch;
     a.block = S->block;
     a.grid = S->grid;
     a.stack_depth = S->stack_depth;
     a.chunk = S->cfg.chunk;
     a.apush = S->apush;
-    hipError_t e = launch_serve(a, S->kstreams[i]);
-    if (e == hipSuccess) e = hipEventRecord(S->kevents[i], S->kstreams[i]);
-    if (e == hipSuccess) {
-        S->kused[i] = true;
-        S->launches++;
-    }
-    return e;
+    hipError_t e = launch_zero(S->head, 1, S->stream);
+    if (e == hipSuccess) e = launch_serve(a, S->stream);
+    if (e == hipSuccess) e = hipEventRecord(S->kev, S->stream);
+    if (e != hipSuccess) return e;
+    S->base = a.base;
+    S->claimed_seen = a.base;
+    S->running = true;
+    S->stopping = false;
+    S->launches++;
+    if (!S->trace_path.empty())
+        S->trace.push_back({now - S->t_open, 'L', a.epoch, a.base, S->tail, 0});
+    return hipSuccess;
 }
 
 // Collect counted slices from the kernel's completion records; a request is done when its last
@@ -301,6 +334,8 @@ static bool collect(nq_server *S, double now) {
         r.count += __atomic_load_n(&S->done[s].count, __ATOMIC_RELAXED);
         r.outstanding--;
         sl->counted = true;
+        if (!S->trace_path.empty())
+            S->trace.push_back({now - S->t_open, 'C', s, sl->seq, sl->end - sl->begin, 0});
         S->slot_slice[s] = nullptr;
         S->free_slots.push_back(static_cast<int>(s));
         S->inflight--;
@@ -323,20 +358,12 @@ static void scheduler(nq_server *S) {
         bool progressed = collect(S, now);
         if (progressed) S->cv_done.notify_all();
         // Publish slices while the unclaimed published work is within the lookahead.  The claim
-        // cursor is known from below: the kernel's hint in ctl->claimed (stores of two waves may
-        // land out of order) and, when the hint would hold publication back, a read of the device
-        // cursor itself (at most every 100 us), so a stale hint never stops the queue.
-        S->claimed_seen = std::max<uint64_t>(S->claimed_seen, __atomic_load_n(&S->ctl->claimed, __ATOMIC_ACQUIRE));
+        // cursor is known from below through the waves' claim records, scanned when the last
+        // known value would hold publication back (at most every 20 us).
         const double lookahead = 2.0 * S->cfg.batch_work;
-        if (!S->pending.empty() && unclaimed_work(S, S->claimed_seen) >= lookahead && now - S->head_read > 0.1) {
+        if (!S->pending.empty() && unclaimed_work(S, S->claimed_seen) >= lookahead && now - S->head_read > 0.02) {
             S->head_read = now;
-            unsigned long long h = 0;
-            hipError_t e = hipMemcpyAsync(&h, S->head, sizeof h, hipMemcpyDeviceToHost, S->copy_stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(S->copy_stream);
-            if (e == hipSuccess)
-                S->claimed_seen = std::max<uint64_t>(S->claimed_seen, h);
-            else
-                set_error(S, NQ_ERR_HIP, std::string("serving cursor read failed: ") + hipGetErrorString(e));
+            S->claimed_seen = std::max<uint64_t>(S->claimed_seen, claimed_max(S));
         }
         Slice sl;
         if (S->error_code == NQ_OK && unclaimed_work(S, S->claimed_seen) < lookahead && pick_slice(S, now, &sl)) {
@@ -345,29 +372,28 @@ static void scheduler(nq_server *S) {
             S->free_slots.pop_back();
             sl.seq = S->next_seq++;
             const size_t first = r.next;
-            r.next += sl.end - sl.begin;
+            r.next += sl.ntasks;
             r.outstanding++;
             if (r.t_start < 0) r.t_start = now;
-            S->pending_tasks -= sl.end - sl.begin;
+            S->pending_tasks -= sl.ntasks;
             if (r.next >= r.tasks.size())
                 S->pending.erase(std::remove(S->pending.begin(), S->pending.end(), sl.req), S->pending.end());
             lk.unlock();
-            const hipError_t e = copy_slice(S, sl, first);
+            const double t_w = now_ms();
+            write_slice(S, sl, first);
             lk.lock();
-            if (e != hipSuccess) {
-                set_error(S, NQ_ERR_HIP, std::string("serving copy failed: ") + hipGetErrorString(e));
-            } else {
-                S->published.push_back(sl);
-                S->slot_slice[sl.slot] = &S->published.back();
-                S->inflight++;
-                S->tail = sl.end;
-                __atomic_store_n(&S->ctl->tail, S->tail, __ATOMIC_RELEASE);
-                S->slices++;
-                progressed = true;
-            }
+            if (!S->trace_path.empty())
+                S->trace.push_back({t_w - S->t_open, 'P', sl.begin, sl.end, S->claimed_seen, now_ms() - t_w});
+            S->published.push_back(sl);
+            S->slot_slice[sl.slot] = &S->published.back();
+            S->inflight++;
+            S->tail = sl.end;
+            __atomic_store_n(&S->ctl->tail[S->parity], S->tail, __ATOMIC_RELEASE);
+            S->slices++;
+            progressed = true;
         }
         if (S->error_code == NQ_OK) {
-            const hipError_t e = maybe_launch(S, now_ms());
+            const hipError_t e = manage_instance(S, now_ms());
             if (e != hipSuccess) set_error(S, NQ_ERR_HIP, std::string("serving kernel failed: ") + hipGetErrorString(e));
         }
         if (S->inflight > 0 && S->busy_since < 0) S->busy_since = now;
@@ -385,7 +411,7 @@ static void scheduler(nq_server *S) {
             S->pending.clear();
             S->cv_done.notify_all();
         }
-        if (S->stop && S->pending.empty() && S->inflight == 0) break;
+        if (S->stop && (S->error_code != NQ_OK || (S->pending.empty() && S->inflight == 0))) break;
         if (S->stop && now - S->t_stop > CLOSE_DRAIN_MS) {
             // Requests still in flight long after close: give up on them rather than wait forever.
             set_error(S, NQ_ERR_STATE, "server closed with requests in flight");
@@ -393,23 +419,25 @@ static void scheduler(nq_server *S) {
             S->cv_done.notify_all();
             break;
         }
-        if (!progressed) S->cv_work.wait_for(lk, std::chrono::microseconds(S->inflight ? 20 : 2000));
+        if (!progressed)
+            S->cv_work.wait_for(lk, std::chrono::microseconds(S->inflight ? 20 : (S->running && !S->stopping) ? 50 : 2000));
     }
-    // Let running instances see the stop flag and drain.
-    __atomic_store_n(&S->ctl->stop, 1u, __ATOMIC_RELEASE);
+    // Stop the running instance (after an error, with tasks in flight: it counts every published
+    // task it has taken or can still take, then exits).
+    if (S->running && !S->stopping) stop_instance(S);
 }
 
 extern "C" {
 
 int nq_server_config_default(nq_server_config *cfg) {
     if (!cfg) return fail(NQ_ERR_ARG, "cfg is NULL");
-    cfg->slots = 4;                    // streams for kernel instances
-    cfg->slot_tasks = 1u << 25;        // ring capacity (tasks; 512 MB)
+    cfg->waves_per_cu = 0;             // as many as fit
+    cfg->slot_tasks = 1u << 23;        // ring capacity (tasks; 128 MB of pinned host memory)
     cfg->slot_requests = 4096;         // slices with tasks in the ring at once
     cfg->max_pending_tasks = 1ull << 26;
     cfg->depth_target = 12;
     cfg->block = 128;
-    cfg->chunk = 16;
+    cfg->chunk = SERVE_BUF;
     cfg->batch_work = 1e10;
     return NQ_OK;
 }
@@ -420,7 +448,7 @@ int nq_server_open(const nq_server_config *cfg_in, nq_server **out) {
     nq_server_config cfg;
     nq_server_config_default(&cfg);
     if (cfg_in) {
-        if (cfg_in->slots) cfg.slots = cfg_in->slots;
+        if (cfg_in->waves_per_cu) cfg.waves_per_cu = cfg_in->waves_per_cu;
         if (cfg_in->slot_tasks) cfg.slot_tasks = cfg_in->slot_tasks;
         if (cfg_in->slot_requests) cfg.slot_requests = cfg_in->slot_requests;
         if (cfg_in->max_pending_tasks) cfg.max_pending_tasks = cfg_in->max_pending_tasks;
@@ -429,9 +457,9 @@ int nq_server_open(const nq_server_config *cfg_in, nq_server **out) {
         if (cfg_in->chunk) cfg.chunk = cfg_in->chunk;
         if (cfg_in->batch_work > 0) cfg.batch_work = cfg_in->batch_work;
     }
-    if (cfg.slots < 1 || cfg.slots > 16 || cfg.slot_requests < 1 || cfg.slot_requests > (1 << 20) ||
+    if (cfg.waves_per_cu < 0 || cfg.slot_requests < 1 || cfg.slot_requests > (1 << 20) ||
         cfg.depth_target < 2 || cfg.depth_target > 26 || cfg.block < 64 || cfg.block > MAX_BLOCK ||
-        cfg.block % 64 || cfg.chunk < 1 || cfg.chunk > 4096 || cfg.slot_tasks < 1 || cfg.slot_tasks > (1ull << 30))
+        cfg.block % 64 || cfg.chunk < 1 || cfg.chunk > SERVE_BUF || cfg.slot_tasks < 1 || cfg.slot_tasks > (1ull << 30))
         return fail(NQ_ERR_ARG, "invalid server configuration");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(NQ_ERR_NO_DEVICE, "Failed to find any gpu!");
@@ -443,33 +471,24 @@ int nq_server_open(const nq_server_config *cfg_in, nq_server **out) {
     hipError_t e = hipGetDevice(&S->device);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&S->cus, hipDeviceAttributeMultiprocessorCount, S->device);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&S->wall_khz, hipDeviceAttributeWallClockRate, S->device);
-    if (e == hipSuccess) e = hipMalloc(&S->ring, S->ring_cap * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc(&S->head, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&S->error, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&S->remaining, nslots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&S->finished, nslots * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&S->count, nslots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&S->expect, nslots * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&S->seq, nslots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc(&S->ring, S->ring_cap * sizeof(uint4), hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostMalloc(&S->ctl, sizeof(ServeCtl), hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostMalloc(&S->done, nslots * sizeof(ServeDone), hipHostMallocCoherent);
-    if (e == hipSuccess) e = hipHostMalloc(&S->staging, STAGING_TASKS * sizeof(uint4), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc(&S->slot_init, 4 * sizeof(unsigned long long), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&S->copy_ev, hipEventDisableTiming);
-    S->kstreams.assign(cfg.slots, nullptr);
-    S->kevents.assign(cfg.slots, nullptr);
-    S->kused.assign(cfg.slots, false);
-    for (int i = 0; i < cfg.slots && e == hipSuccess; ++i) {
-        e = hipStreamCreateWithFlags(&S->kstreams[i], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&S->kevents[i], hipEventDisableTiming);
-    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&S->kev, hipEventDisableTiming);
     if (e == hipSuccess) {
-        // Zero the claim cursor, error word and slot counters with kernels (no memset).
-        e = launch_zero(S->head, 1, S->copy_stream);
-        if (e == hipSuccess) e = launch_zero(S->error, 1, S->copy_stream);
-        if (e == hipSuccess) e = launch_zero(S->remaining, nslots, S->copy_stream);
-        if (e == hipSuccess) e = launch_zero(S->count, nslots, S->copy_stream);
-        if (e == hipSuccess) e = launch_zero(S->seq, nslots, S->copy_stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(S->copy_stream);
+        // Zero the error word and slot counters with kernels (no memset).
+        e = launch_zero(S->error, 1, S->stream);
+        if (e == hipSuccess) e = launch_zero(S->finished, nslots, S->stream);
+        if (e == hipSuccess) e = launch_zero(S->count, nslots, S->stream);
+        if (e == hipSuccess) e = launch_zero(S->expect, nslots, S->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
     }
     if (e == hipSuccess) {
         const int frames = std::max(1, cfg.depth_target - 1);  // tasks keep at most depth_target free rows
@@ -477,17 +496,23 @@ int nq_server_open(const nq_server_config *cfg_in, nq_server **out) {
         e = serve_fit(cfg.block, frames, 0, &S->apush, &bpc);
         S->block = cfg.block;
         S->stack_depth = frames;
+        if (cfg.waves_per_cu > 0) bpc = std::min(bpc, std::max(1, cfg.waves_per_cu / (cfg.block / 64)));
         S->grid = std::max(1, S->cus * std::max(1, bpc));
+        S->waves = static_cast<uint64_t>(S->grid) * (cfg.block / 64);
     }
+    if (e == hipSuccess)
+        e = hipHostMalloc(&S->claimed, S->waves * sizeof(unsigned long long), hipHostMallocCoherent);
     if (e != hipSuccess) {
         free_server(S);
         return fail_hip(e, "server allocation");
     }
     memset(S->ctl, 0, sizeof(ServeCtl));
     memset(S->done, 0, nslots * sizeof(ServeDone));
+    memset(S->claimed, 0, S->waves * sizeof(unsigned long long));
     S->slot_slice.assign(nslots, nullptr);
     for (int s = static_cast<int>(nslots) - 1; s >= 0; --s) S->free_slots.push_back(s);
     S->t_open = now_ms();
+    if (const char *tp = getenv("NQ_SERVE_TRACE")) S->trace_path = tp;
     S->thr = std::thread(scheduler, S);
     *out = S;
     return NQ_OK;
@@ -501,10 +526,10 @@ int nq_server_submit(nq_server *S, const nq_instance *inst, uint64_t *ticket) {
     req->N = inst->N;
     const int d0 = append_instance_tasks(inst->N, inst->k, c, l, r, S->cfg.depth_target, 0, req->tasks);
     req->task_work = est_task_nodes(inst->N - d0);
-    // A request that can never fit the ring is refused for good (NQ_ERR_NOMEM), unlike a full
-    // queue (NQ_ERR_BUSY), which a retry may pass.
-    if (req->tasks.size() > S->ring_cap)
-        return fail(NQ_ERR_NOMEM, "request larger than the server's ring (raise slot_tasks or depth_target)");
+    // A request that can never fit the admission budget is refused for good (NQ_ERR_NOMEM), unlike
+    // a full queue (NQ_ERR_BUSY), which a retry may pass.  (The ring takes it slice by slice.)
+    if (req->tasks.size() > S->cfg.max_pending_tasks)
+        return fail(NQ_ERR_NOMEM, "request larger than the admission budget (raise max_pending_tasks or depth_target)");
     std::lock_guard<std::mutex> g(S->mu);
     if (S->stop) return fail(NQ_ERR_STATE, "server is closing");
     if (S->pending_tasks + req->tasks.size() > S->cfg.max_pending_tasks)
@@ -590,12 +615,20 @@ int nq_server_close(nq_server *S) {
         S->cv_work.notify_all();
     }
     if (S->thr.joinable()) S->thr.join();
-    // Every instance sees the stop flag within its idle wait; wait for them to drain.
-    for (size_t i = 0; i < S->kstreams.size(); ++i) (void)hipStreamSynchronize(S->kstreams[i]);
+    // The scheduler has stopped the instance; it exits once its waves see that.
+    (void)hipStreamSynchronize(S->stream);
     unsigned long long err = 0;
     (void)hipMemcpy(&err, S->error, sizeof err, hipMemcpyDeviceToHost);
     int rc = S->error_code;
     if (!rc && err) rc = (err & DEV_ERR_LDS) ? NQ_ERR_STATE : NQ_ERR_DEPTH;
+    if (!S->trace_path.empty()) {
+        if (FILE *f = fopen(S->trace_path.c_str(), "w")) {
+            for (const TraceEvent &ev : S->trace)
+                fprintf(f, "%.3f %c %llu %llu %llu %.3f\n", ev.t, ev.kind, static_cast<unsigned long long>(ev.a),
+                        static_cast<unsigned long long>(ev.b), static_cast<unsigned long long>(ev.c), ev.d);
+            fclose(f);
+        }
+    }
     free_server(S);
     return rc ? fail(rc, "server had a device error") : NQ_OK;
 }
