@@ -9,8 +9,11 @@ placed at depths rows+1..N-1).  Also reported:
   * n21_wall_s: the whole `n_queens 21 6` CLI call (generation, device init, copies, kernel) —
     the reference's own measure (src/main.cpp:49-54; 21.242 s on one RTX 5090, README.md:24);
   * roofline: integer-VALU bound of the DFS kernel (11 int32 ops per node);
-  * cpu_baseline: the reference's own OpenMP path (oracle/_ref, parallel_n_queens) timed on this
-    host on a bounded sample.
+  * cpu_baseline: the reference's own OpenMP path (oracle/_ref, parallel_n_queens, which fixes
+    32 threads) timed on this host on a bounded sample, and cpu_baseline_port: the oracle's
+    restatement on every usable CPU of this host (one thread per CPU), same sample;
+  * batched_mix and serving: the 512-instance mix and the 400 Poisson requests, every count
+    checked against the oracle's (tests/golden/instance_counts.json).
 Prints ONE JSON line on stdout.
 """
 from __future__ import annotations
@@ -40,17 +43,8 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nodes_fn) -> dict | None:
-    """The reference's OpenMP path (32 threads, src/n_queens.cpp:139-163) on N=18 rows 6."""
-    import oracle
-    R = oracle.ref()
-    if R is None:
-        return None
-    N, rows = 18, 6
-    t0 = time.perf_counter()
-    total = int(R.ref_parallel_n_queens(N, rows))
-    dt = time.perf_counter() - t0
-    nodes = nodes_fn(N, rows)
+def host_cpus() -> tuple[int, str]:
+    """Usable CPUs of this process and the CPU model name."""
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     model = ""
     try:
@@ -60,11 +54,43 @@ def cpu_baseline(nodes_fn) -> dict | None:
                 break
     except OSError:
         pass
-    return {"value": nodes / dt, "unit": "nodes/s", "cores": 32, "kind": "reference",
-            "sample": f"whole N={N} board, {rows} rows pre-placed, reference parallel_n_queens "
-                      f"(omp_set_num_threads(32)) in {dt:.2f} s, {total} solutions, {nodes} nodes; "
-                      f"host: {ncpu} usable CPUs, {model}",
+    return ncpu, model
+
+
+def cpu_baselines(nodes_fn) -> tuple[dict | None, dict]:
+    """Whole N=18 board with 6 rows pre-placed on the host: the reference's OpenMP path
+    (src/n_queens.cpp:139-163, omp_set_num_threads(32)) and the oracle's restatement with one
+    thread per usable CPU."""
+    import oracle
+    N, rows = 18, 6
+    ncpu, model = host_cpus()
+    nodes = nodes_fn(N, rows)
+    host = f"host: {ncpu} usable CPUs, {model}"
+    ref = None
+    R = oracle.ref()
+    if R is not None:
+        t0 = time.perf_counter()
+        total = int(R.ref_parallel_n_queens(N, rows))
+        dt = time.perf_counter() - t0
+        ref = {"value": nodes / dt, "unit": "nodes/s", "cores": ncpu, "threads": 32, "kind": "reference",
+               "sample": f"whole N={N} board, {rows} rows pre-placed: the reference's parallel_n_queens "
+                         f"(32 OpenMP threads, as it fixes them) in {dt:.2f} s, {total} solutions, {nodes} "
+                         f"DFS nodes; {host}",
+               "seconds": dt, "solutions": total}
+    t0 = time.perf_counter()
+    total = oracle.parallel_n_queens(N, rows, threads=ncpu)
+    dt = time.perf_counter() - t0
+    port = {"value": nodes / dt, "unit": "nodes/s", "cores": ncpu, "threads": ncpu, "kind": "port",
+            "sample": f"whole N={N} board, {rows} rows pre-placed: the oracle's parallel_n_queens restatement "
+                      f"(one OpenMP thread per usable CPU) in {dt:.2f} s, {total} solutions; {host}",
             "seconds": dt, "solutions": total}
+    return ref, port
+
+
+def instance_fixture(name: str) -> dict:
+    """The oracle's count of every instance of a seeded workload (tests/golden/instance_counts.json)."""
+    with open(os.path.join(ROOT, "tests", "golden", "instance_counts.json")) as f:
+        return json.load(f)[name]
 
 
 def batched_mix(nq, log) -> dict:
@@ -90,25 +116,37 @@ def batched_mix(nq, log) -> dict:
     t_replay = (time.perf_counter() - t0) / reps
     plan.close()
     same = [int(x) for x in counts] == per and [int(x) for x in c2] == per
+    fx = instance_fixture("batch_mix")
+    oracle_ok = fx["digest"] == workloads.instances_digest(mix) and per == fx["counts"]
     out = {"instances": n, "per_launch_inst_per_s": n / t_per, "batch_inst_per_s": n / t_batch,
            "batch_replay_inst_per_s": n / t_replay, "speedup": t_per / t_batch,
-           "speedup_replay": t_per / t_replay, "counts_identical": same, "per_launch_s": t_per,
+           "speedup_replay": t_per / t_replay, "counts_identical": same,
+           "counts_match_oracle_fixture": oracle_ok, "per_launch_s": t_per,
            "batch_s": t_batch, "batch_replay_s": t_replay, "batch_kernel_ms": st2["kernel_ms"],
            "ntasks": int(st["ntasks"]), "solutions": int(sum(per)),
            "workload": "512 instances, N uniform in [14,20], 2-6 fixed rows, seed 0x6E51"}
     log(f"batched mix: per-launch {t_per:.3f}s  batch {t_batch:.3f}s  replay {t_replay:.3f}s  "
-        f"speedup {t_per / t_batch:.2f}x (replay {t_per / t_replay:.2f}x)  identical={same}")
+        f"speedup {t_per / t_batch:.2f}x (replay {t_per / t_replay:.2f}x)  identical={same} oracle={oracle_ok}")
     return out
 
 
 def serving(nq, log, n_req: int = 400, util: float = 0.7) -> dict:
     """Poisson arrivals (seeded) of requests with N in [12,19] and a 0-5 row prefix, at a rate
-    set to `util` of the measured service capacity; p50/p99 latency and solutions/s; every
-    served count checked against the same request counted alone."""
+    set to `util` of the measured service capacity (all requests offered at once); p50/p99
+    latency, the same split into whole boards and prefixed requests, slowdown (latency over the
+    request's own time counted alone with one launch), solutions/s; every served count checked
+    against the same request counted alone and against the oracle's."""
     import threading
     from n_queens_amd import workloads
     reqs0 = workloads.poisson_requests(n_req, lam=1.0)
-    alone = [nq.instance_count(N, cols) for _, N, cols in reqs0]
+    nq.instance_count(*reqs0[0][1:])  # warm the one-launch path
+    alone, alone_ms = [], []
+    for _, N, cols in reqs0:
+        t0 = time.perf_counter()
+        alone.append(nq.instance_count(N, cols))
+        alone_ms.append((time.perf_counter() - t0) * 1e3)
+    fx = instance_fixture("poisson")
+    oracle_ok = fx["digest"] == workloads.instances_digest([(N, cols) for _, N, cols in reqs0]) and alone == fx["counts"]
     busy = [0]
 
     def submit(srv, N, cols, deadline_s: float = 60.0):
@@ -157,10 +195,20 @@ def serving(nq, log, n_req: int = 400, util: float = 0.7) -> dict:
     lat = np.array([x["latency_ms"] for _, x in res])
     served = [c for c, _ in res]
     same = served == alone and burst == alone
+    slow = lat / np.maximum(np.array(alone_ms), 1e-3)
+    whole = np.array([not cols for _, _, cols in reqs0])
+
+    def pct(x):
+        return {"n": int(len(x)), "p50_ms": float(np.percentile(x, 50)), "p99_ms": float(np.percentile(x, 99))}
     out = {"requests": n_req, "capacity_req_per_s": capacity, "lambda_req_per_s": lam, "target_util": util,
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
            "mean_ms": float(lat.mean()), "solutions_per_s": sum(served) / span, "req_per_s": n_req / span,
-           "batches": int(st["batches"]), "counts_identical_alone_vs_mix": same, "span_s": span,
+           "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+           "slowdown_p50": float(np.percentile(slow, 50)), "slowdown_p99": float(np.percentile(slow, 99)),
+           "whole_boards": pct(lat[whole]), "prefixed": pct(lat[~whole]),
+           "alone_total_s": sum(alone_ms) / 1e3,
+           "batches": int(st["batches"]), "counts_identical_alone_vs_mix": same,
+           "counts_match_oracle_fixture": oracle_ok and served == fx["counts"], "span_s": span,
            "busy_retries": busy[0], "busy_retries_capacity_probe": busy_burst,
            "device_busy_ms": st["device_ms"],
            "workload": "Poisson arrivals, N uniform in [12,19], 0-5 fixed rows, seed 0x5E7E"}
@@ -342,11 +390,16 @@ def main() -> int:
         def nodes_fn(n, r):
             t = nq.generate(n, r)
             return nq.count(n, t, nodes=True).nodes
-        cb = cpu_baseline(nodes_fn)
-        if cb:
-            out["cpu_baseline"] = cb
-            out["cpu_est_wall_s"] = nodes / cb["value"]
-            log(f"cpu baseline {cb['value']:.3e} nodes/s ({cb['seconds']:.1f} s)")
+        ref, port = cpu_baselines(nodes_fn)
+        out["cpu_baseline"] = ref or port
+        out["cpu_baseline_port"] = port
+        # Not measured: the N=21 count's node total over each baseline's rate.
+        out["cpu_n21_estimate_s" if N == 21 else "cpu_estimate_s"] = {
+            "reference_32_threads": nodes / ref["value"] if ref else None, "port_all_cpus": nodes / port["value"],
+            "basis": "estimate: this count's DFS nodes / the sample's nodes per second (not timed)"}
+        for b in (ref, port):
+            if b:
+                log(f"cpu baseline ({b['kind']}, {b['threads']} threads) {b['value']:.3e} nodes/s ({b['seconds']:.1f} s)")
     print(json.dumps(out), flush=True)
     return 0
 

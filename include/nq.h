@@ -195,12 +195,13 @@ int nq_instance_count(const nq_instance *inst, const nq_config *cfg, uint64_t *c
  * A server keeps one ring of tasks, in pinned host memory that the device reads directly, which a
  * persistent kernel consumes while the server keeps appending to it (publishing is plain stores:
  * no copy ever waits behind the running kernel).  nq_server_submit validates a request, expands it into tasks on
- * the caller's thread (until no task keeps more than depth_target free rows) and admits it
+ * the caller's thread (until no task keeps more than depth_target free rows, and a small request
+ * further, into at least 4096 tasks unless they would keep fewer than 5 free rows) and admits it
  * while the pending work fits max_pending_tasks (else NQ_ERR_BUSY, which a later retry may
  * pass; a request of more tasks than max_pending_tasks can never fit and is refused with
  * NQ_ERR_NOMEM).  A
  * scheduler thread publishes requests into the ring in slices of at most batch_work estimated
- * DFS nodes, least estimated remaining work first (aged by waiting time), keeping no more than
+ * DFS nodes, least estimated remaining work first (aged after a second of waiting), keeping no more than
  * two slices' work published and unclaimed, so that requests arriving while others run are
  * admitted into the running kernel within that much work.  The kernel counts every task into
  * its request's slot and publishes a request's count as soon as its last task is counted.  A
@@ -217,7 +218,7 @@ typedef struct nq_server_config {
     int block;                   /* workgroup size (default 128)                           */
     int chunk;                   /* tickets a wave claims at once while published work is  */
                                  /* plentiful, 1..64 (default 64); else one per idle lane  */
-    double batch_work;           /* estimated DFS nodes per published slice (default 1e10) */
+    double batch_work;           /* estimated DFS nodes per published slice (default 1e9)  */
 } nq_server_config;
 
 typedef struct nq_request_stats {

@@ -15,6 +15,7 @@
 #include "nq.h"
 #include "nq_internal.h"
 #include "nq_kernels.h"
+#include "nq_tasks.h"
 
 namespace nq {
 
@@ -45,56 +46,21 @@ bool encode(const nq_instance &in, Prefix *out) {
 // Rows to expand on the host so that no task needs more than `depth_target` levels.
 int split_rows(int N, int k, int depth_target) { return std::max(0, (N - k) - depth_target); }
 
-// All placements of `rows` more queens below (c, l, r), no symmetry restriction, appended as
-// {c, l, r, tag}.
-void expand(int N, uint32_t c, uint32_t l, uint32_t r, int rows, uint32_t tag, std::vector<uint4> &out) {
-    if (rows <= 0) {
-        out.push_back({c, l, r, tag});
-        return;
-    }
-    const uint32_t last = (1u << N) - 1u;
-    struct F {
-        uint32_t c, l, r, v;
-    };
-    F st[33];
-    int top = 0;
-    st[0] = {c, l, r, last & ~(c | l | r)};
-    while (top >= 0) {
-        F &f = st[top];
-        if (f.v == 0) {
-            --top;
-            continue;
-        }
-        const uint32_t p = f.v & (0u - f.v);
-        f.v ^= p;
-        const uint32_t c2 = f.c | p, l2 = (f.l | p) << 1, r2 = (f.r | p) >> 1;
-        if (top + 1 == rows) {
-            out.push_back({c2, l2, r2, tag});
-            continue;
-        }
-        ++top;
-        st[top] = {c2, l2, r2, last & ~(c2 | l2 | r2)};
-    }
-}
-
 }  // namespace
 
-// Tasks of one instance, tagged idx << 6 | twice << 5 | N; returns their prefix depth.  An empty
-// board (k = 0, N >= 4) is split with the reference's mirror-symmetric generator
-// (src/n_queens.cpp:74-113) and its tasks carry weight 2, halving its work; every other prefix
-// is expanded plainly.
 int append_instance_tasks(int N, int k, uint32_t c, uint32_t l, uint32_t r, int depth_target, uint32_t idx,
-                          std::vector<uint4> &out) {
+                          TaskVec &out) {
     const int s = split_rows(N, k, depth_target);
     if (k == 0 && N >= 4) {
+        // The reference's mirror-symmetric task list (generate_tasks' order), weight 2.
         const int rows = std::max(2, s);
-        std::vector<uint32_t> t;
-        generate_tasks(N, rows, t);
         const uint32_t tag = (idx << 6) | (1u << 5) | static_cast<uint32_t>(N);
-        for (size_t i = 0; i + 2 < t.size(); i += 3) out.push_back({t[i], t[i + 1], t[i + 2], tag});
+        const ExpandPart parts[2] = {{EXPAND_EVEN, 0, 0, 0}, {EXPAND_ODD, 0, 0, 0}};
+        expand_tagged(N, parts, (N & 1) ? 2 : 1, rows, tag, out);
         return rows;
     }
-    expand(N, c, l, r, s, (idx << 6) | static_cast<uint32_t>(N), out);
+    const ExpandPart part{EXPAND_PLAIN, c, l, r};
+    expand_tagged(N, &part, 1, s, (idx << 6) | static_cast<uint32_t>(N), out);
     return k + s;
 }
 
@@ -162,7 +128,7 @@ int build_batch_seeds(const nq_instance *inst, int n, int depth_target, std::vec
 int instance_tasks(const nq_instance &in, int depth_target, std::vector<uint32_t> &tasks, int *weight) {
     Prefix p;
     if (!encode(in, &p)) return fail(NQ_ERR_ARG, "invalid N/k or attacking prefix");
-    std::vector<uint4> t;
+    TaskVec t;
     append_instance_tasks(in.N, in.k, p.c, p.l, p.r, depth_target, 0, t);
     tasks.resize(3 * t.size());
     for (size_t i = 0; i < t.size(); ++i) {

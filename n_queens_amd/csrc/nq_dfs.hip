@@ -666,27 +666,24 @@ __global__ void __launch_bounds__(MAX_BLOCK) serve_kernel(ServeArgs a) {
     uint32_t weight = 1;
     uint64_t tsum = 0;
     bool own = false;  // the lane holds a task it is counting
-    // The lane's run of counted entries of one slot, added with one atomic each when the lane
-    // takes an entry of another slot or waits, so that a slice completes with its last task.
+    // The lane's run of counted entries of one slot (their count and weight), added with one
+    // fetch-and-add when the lane takes an entry of another slot or waits, so that a slice
+    // completes with its last entry.
     uint32_t run_slot = 0;
     uint64_t run_sum = 0, run_n = 0;
     auto flush = [&]() {
         if (!run_n) return;
-        atomicAdd(&a.count[run_slot], static_cast<unsigned long long>(run_sum));
-        const uint64_t before = __hip_atomic_fetch_add(&a.finished[run_slot], static_cast<unsigned long long>(run_n),
-                                                       __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t want = __hip_atomic_load(&a.expect[run_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (before + run_n == want) {
-            // The slice's last entry: every other entry's count and the terminator's stores are
-            // visible (acquire above).  Reset the slot for its next slice (the host reuses it only
-            // after it has seen the record), then publish the count and, last, the sequence number.
-            const uint64_t total = __hip_atomic_load(&a.count[run_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint64_t q = __hip_atomic_load(&a.seq[run_slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&a.count[run_slot], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&a.finished[run_slot], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&a.expect[run_slot], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&a.done[run_slot].count, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            __hip_atomic_store(&a.done[run_slot].seq, q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint64_t add = (run_sum << SERVE_FIN_BITS) | run_n;
+        const uint64_t now = __hip_atomic_fetch_add(&a.acc[run_slot], static_cast<unsigned long long>(add),
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + add;
+        if ((now & ((1ull << SERVE_FIN_BITS) - 1)) == SERVE_FIN_DONE) {
+            // The slice's last entry: `now` holds its whole count (one word, so nothing else to
+            // wait for).  Clear the word before the record can let the host reuse the slot (the
+            // exchange returns once done, and the record's release follows it).
+            (void)__hip_atomic_exchange(&a.acc[run_slot], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t g = __hip_atomic_fetch_add(&a.gen[run_slot], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+            __hip_atomic_store(&a.done[run_slot].count, now >> SERVE_FIN_BITS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&a.done[run_slot].gen, g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         run_sum = 0;
         run_n = 0;
@@ -744,13 +741,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) serve_kernel(ServeArgs a) {
                 const uint32_t v0 = last & ~(w.x | w.y | w.z);
                 const uint32_t d0 = __popc(w.x);
                 if (w.w & SERVE_TAG_TERM) {
-                    // The slice's terminator: its entry count and sequence number, stored before
-                    // this lane's own count (the flush's release orders them).
-                    __hip_atomic_store(&a.seq[s], (static_cast<uint64_t>(w.z) << 32) | w.y, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(&a.expect[s], static_cast<unsigned long long>(w.x), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    run_n += 1;
+                    // The slice's terminator weighs what completes the entry field.
+                    run_n += SERVE_FIN_DONE - w.x;
                 } else if (d0 + 1u >= N || v0 == 0) {
                     const uint64_t direct =
                         d0 >= N ? (w.x == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);

@@ -18,6 +18,11 @@ void generate_tasks(int N, int rows, std::vector<uint32_t> &out);
 void generate_from(int N, bool odd_part, uint32_t cur, uint32_t left, uint32_t right, int rows,
                    std::vector<uint32_t> &out);
 
+// Modes of expand_tagged (nq_prefix.cpp; declared where HIP's uint4 is known): EXPAND_PLAIN
+// places queens anywhere legal, EXPAND_EVEN and EXPAND_ODD apply the row restrictions of the
+// reference's two generators (generate_tasks' parts).
+enum { EXPAND_PLAIN = 0, EXPAND_EVEN = 1, EXPAND_ODD = 2 };
+
 // nq_solve with the task list generated on the device (nq_api.cpp); on_generated (nullable)
 // receives the reference's task count for (N, rows) and the generation time in ms.
 int solve_on_device(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *stats,

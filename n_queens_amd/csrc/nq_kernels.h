@@ -64,8 +64,8 @@ struct BatchArgs {
 // kernel writes back.
 //
 // Ring entries are 16 bytes: {cur, left, right, tag}, tag = slot << 7 | TERM | twice << 5 | N.
-// A published slice is its tasks followed by one terminator entry (tag bit SERVE_TAG_TERM,
-// x = tasks + 1, y/z = the slice's sequence number, low/high).
+// A published slice is its tasks (at most SERVE_SLICE_MAX) followed by one terminator entry
+// (tag bit SERVE_TAG_TERM, x = the slice's task count).
 //
 // Claims are tickets: a wave takes tickets with one fetch-and-add on the device `head` (a claim
 // never fails or retries, however many waves claim at once) and loads their entries with one
@@ -74,10 +74,14 @@ struct BatchArgs {
 // raise the tail.  Each wave records the end of its latest claim in its `claimed` entry (epoch in
 // bits 48..63, end - base below), which the host reads as the claim cursor.
 //
-// Every finished task adds its count to count[slot] and one to finished[slot]; the lane that
-// takes a terminator stores its total into expect[slot] and its sequence number into seq[slot]
-// (device memory) before it adds its own one.  Whoever brings finished[slot] to expect[slot]
-// resets count, finished and expect to zero and writes {count, seq} into done[slot].
+// Each slot has one 64-bit accumulator acc[slot] = count << SERVE_FIN_BITS | entries: a lane
+// adds a run of finished entries of one slot with one fetch-and-add of (their count << FIN_BITS |
+// their weight), where a task weighs 1 and the terminator SERVE_FIN_DONE - tasks, so the entry
+// field reaches SERVE_FIN_DONE exactly when the slice's last entry is added (and never passes it:
+// no carry into the count).  The lane whose add brings it there takes the slice's count from
+// the same word, clears it (waiting for that before it writes the record), numbers the
+// completion with gen[slot] and writes {count, generation} into done[slot].  The host sizes slices
+// so that their count fits the field (SERVE_COUNT_BITS).
 //
 // The host stops an instance only with every published task counted: it stores the instance's
 // epoch into stop_epoch and from then on publishes into the other tail, so the stopped instance
@@ -89,6 +93,10 @@ constexpr uint32_t SERVE_TAG_TERM = 1u << 6;
 constexpr int SERVE_TAG_SLOT_SHIFT = 7;
 constexpr int SERVE_CLAIM_EPOCH_SHIFT = 48;  // claimed[] entries: epoch << 48 | (end - base)
 constexpr int SERVE_BUF = 64;                // tickets a wave claims and loads at most at once
+constexpr int SERVE_FIN_BITS = 18;
+constexpr unsigned long long SERVE_SLICE_MAX = 1ull << 16;           // tasks per slice
+constexpr unsigned long long SERVE_FIN_DONE = 2 * SERVE_SLICE_MAX;   // entry field of a counted slice
+constexpr int SERVE_COUNT_BITS = 64 - SERVE_FIN_BITS;
 
 struct ServeCtl {
     unsigned long long tail[2];  // tasks published so far, per instance parity (host)
@@ -97,8 +105,8 @@ struct ServeCtl {
 };
 
 struct ServeDone {
-    unsigned long long count;  // the slice's count once seq is stored
-    unsigned long long seq;    // the slice's sequence number, stored last
+    unsigned long long count;  // the slice's count once gen is stored
+    unsigned long long gen;    // completions of this slot so far, stored last
 };
 
 struct ServeArgs {
@@ -109,10 +117,8 @@ struct ServeArgs {
     const ServeCtl *ctl;            // host
     ServeDone *done;                // host, one per slot
     unsigned long long *claimed;    // host, one per wave: epoch and end of its latest claim
-    unsigned long long *finished;   // device, one per slot: entries counted (zero between slices)
-    unsigned long long *count;      // device, one per slot (zero between slices)
-    unsigned long long *expect;     // device, one per slot: entries of the slice (0: not known yet)
-    unsigned long long *seq;        // device, one per slot: the slice's sequence number
+    unsigned long long *acc;        // device, one per slot: count << FIN_BITS | entries (zero between slices)
+    unsigned long long *gen;        // device, one per slot: completions so far
     unsigned long long *error;      // device: DEV_ERR_* bits
     unsigned long long orphan_ticks;  // wall-clock ticks a wave waits for work before it exits
     unsigned int epoch;             // this launch's number

@@ -7,18 +7,18 @@
 // while the host keeps appending to it:
 //
 //   * it publishes requests in slices (at most batch_work estimated DFS nodes each), least
-//     estimated remaining work first, aged by waiting time, and only while the published but
+//     estimated remaining work first (aged after a second of waiting), and only while the published but
 //     unclaimed work stays under a lookahead of two slices, so a request that arrives behind a
 //     large one waits for at most that much work (every kernel wave records the end of its last
 //     claim in host memory; the largest record is the claim cursor);
 //   * the ring, the tail and the completion records live in pinned, coherent host memory that
 //     the kernel reads and writes directly: publishing a slice is plain stores (its tasks and a
-//     terminator entry carrying its size and sequence number, outside the server lock) followed
-//     by raising the tail, and no copy is ever queued behind the running kernel;
+//     terminator entry carrying its size, outside the server lock) followed by raising the tail,
+//     and no copy is ever queued behind the running kernel;
 //   * every published slice occupies one of slot_requests slots; the kernel adds every finished
-//     task's count to its slot and, when the slice's last entry is counted, writes {count,
-//     sequence number} into host memory, where the scheduler finds it; the slice's ring space is
-//     then free, and a request is done when its last slice is counted;
+//     task's count to its slot and, when the slice's last entry is counted, writes {count, the
+//     slot's completion number} into host memory, where the scheduler finds it; the slice's ring
+//     space is then free, and a request is done when its last slice is counted;
 //   * kernel waves claim tasks as tickets with a fetch-and-add on the device cursor (a claim
 //     never fails, however many waves claim at once) and read their entries with one coalesced
 //     load per claim; a ticket past the published tail waits for the host to publish it;
@@ -29,9 +29,9 @@
 //
 // A request's count (the sum over its tasks) is exactly the one nq_instance_count gives.
 //
-// With NQ_SERVE_TRACE=<file> in the environment the scheduler records its events (slices
-// published, kernel instances launched, instances seen retiring, slices counted) and writes them
-// to <file> at close, one per line: "<ms since open> <event> <fields>".
+// With NQ_SERVE_TRACE=<file> in the environment the server records its events (requests
+// admitted, slices published, kernel instances launched and stopped, slices counted) and writes
+// them to <file> at close, one per line: "<ms since open> <event> <fields>".
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -39,6 +39,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -53,10 +54,9 @@
 #include "nq.h"
 #include "nq_internal.h"
 #include "nq_kernels.h"
+#include "nq_tasks.h"
 
 namespace nq {
-int append_instance_tasks(int N, int k, uint32_t c, uint32_t l, uint32_t r, int depth_target, uint32_t idx,
-                          std::vector<uint4> &out);
 bool encode_instance(const nq_instance &in, uint32_t *c, uint32_t *l, uint32_t *r);
 }  // namespace nq
 
@@ -67,10 +67,11 @@ namespace {
 struct Request {
     uint64_t ticket = 0;
     int N = 0;
-    std::vector<uint4> tasks;  // tag = twice << 5 | N; the slice's slot is added when published
+    TaskVec tasks;             // tag = twice << 5 | N; the slice's slot is added when published
     size_t next = 0;           // first task not yet published
     int outstanding = 0;       // published slices not yet counted
     double task_work = 1;      // estimated DFS nodes per task
+    int free_rows = 0;         // rows every task leaves to the kernel
     double t_submit = 0, t_start = -1, t_done = 0;
     uint64_t count = 0;        // sum of the counted slices
     bool done = false;
@@ -86,7 +87,7 @@ double est_task_nodes(int m) {
 }
 
 // A published slice: ring positions [begin, end) of one request, its estimated work, and the
-// slot the kernel counts it in (every slice has its own slot and sequence number, so that its
+// slot the kernel counts it in (every slice has its own slot and completion number, so that its
 // ring space is freed as soon as it is counted, whatever the rest of its request does).
 struct Slice {
     std::shared_ptr<Request> req;
@@ -94,18 +95,35 @@ struct Slice {
     uint64_t ntasks = 0;
     double work = 0;
     int slot = -1;
-    uint64_t seq = 0;
+    uint64_t seq = 0;  // the slice's number (trace only)
+    uint64_t gen = 0;  // the slot's completion number this slice's record will carry
     bool counted = false;
 };
 
-constexpr uint64_t SLICE_MAX_TASKS = 1u << 20;  // tasks one slice writes into the ring at most
+// Tasks one slice writes into the ring at most (SERVE_SLICE_MAX = 2^16): the scheduler collects
+// completions between slices, so a slice's write time (about 0.5 ms here) bounds how late a
+// finished request is seen.
+
+// Most tasks of `free_rows` free rows and weight `weight` whose completions certainly fit the
+// kernel's count field: a task has at most free_rows! completions.
+uint64_t slice_cap(int free_rows, int weight) {
+    long double most = weight;
+    for (int i = 2; i <= free_rows; ++i) most *= i;
+    const long double cap = std::ldexp(1.0L, SERVE_COUNT_BITS) / most;
+    return cap >= static_cast<long double>(SERVE_SLICE_MAX) ? SERVE_SLICE_MAX
+                                                            : std::max<uint64_t>(1, static_cast<uint64_t>(cap));
+}
+constexpr size_t SMALL_REQUEST_TASKS = 4096;    // a request is split into at least this many tasks
+constexpr int MIN_FREE_ROWS = 5;                // ... unless they would keep fewer free rows
 constexpr double IDLE_STOP_MS = 0.2;          // an idle server stops its kernel after this long
 constexpr double ORPHAN_S = 30;               // a kernel wave without work for this long exits
+constexpr double AGING_AFTER_MS = 1000;       // waiting time after which a request gains priority
 constexpr double CLOSE_DRAIN_MS = 120000;     // close() waits this long for requests in flight
 
 struct TraceEvent {
     double t;
-    char kind;  // 'P' slice published, 'L' instance launched, 'S' instance stopped, 'C' slice counted
+    char kind;  // 'A' request admitted, 'P' slice published, 'L' instance launched, 'S' instance
+                // stopped, 'C' slice counted
     uint64_t a, b, c;
     double d;
 };
@@ -116,8 +134,7 @@ struct nq_server {
     nq_server_config cfg{};
     int device = 0, cus = 0, wall_khz = 0;
     // Device memory.
-    unsigned long long *head = nullptr, *finished = nullptr, *count = nullptr, *expect = nullptr, *seq = nullptr,
-                       *error = nullptr;
+    unsigned long long *head = nullptr, *acc = nullptr, *gen = nullptr, *error = nullptr;
     // Pinned, coherent host memory.
     uint4 *ring = nullptr;
     uint64_t ring_cap = 0;  // a power of two
@@ -141,11 +158,11 @@ struct nq_server {
     std::condition_variable cv_work, cv_done;
     std::vector<std::shared_ptr<Request>> pending;  // requests with tasks not yet published
     std::unordered_map<uint64_t, std::shared_ptr<Request>> all;
-    std::vector<Slice *> slot_slice;  // slice counted in each slot (null: free)
     std::vector<int> free_slots;
     std::deque<Slice> published;  // ring FIFO, freed from the front as slices are counted
     uint64_t tail = 0;            // ring entries published
     uint64_t next_ticket = 1, next_seq = 1, pending_tasks = 0;  // next_seq numbers slices
+    std::vector<uint64_t> slot_gen;  // completions each slot will have had once its slice is counted
     int inflight = 0;  // slices published and not yet counted
     double head_read = -1e30;   // last scan of the waves' claim records (ms)
     uint64_t claimed_seen = 0;  // largest claim record seen (a lower bound of the claim cursor)
@@ -162,7 +179,7 @@ struct nq_server {
 };
 
 static void free_server(nq_server *S) {
-    for (unsigned long long *p : {S->head, S->finished, S->count, S->expect, S->seq, S->error})
+    for (unsigned long long *p : {S->head, S->acc, S->gen, S->error})
         if (p) (void)hipFree(p);
     if (S->ring) (void)hipHostFree(S->ring);
     if (S->ctl) (void)hipHostFree(S->ctl);
@@ -192,8 +209,8 @@ static double unclaimed_work(const nq_server *S, uint64_t claimed) {
 }
 
 // The next slice to publish (lock held): the pending request with the least estimated remaining
-// work, aged by its waiting time; its tasks up to the slice budget, the ring space and
-// SLICE_MAX_TASKS.  Returns false when nothing can be published now (no request, no free slot, no space).
+// work (aged once it has waited AGING_AFTER_MS); its tasks up to the slice budget, the ring space and
+// slice_cap.  Returns false when nothing can be published now (no request, no free slot, no space).
 static bool pick_slice(nq_server *S, double now, Slice *out) {
     if (S->pending.empty() || S->free_slots.empty()) return false;
     const uint64_t front = S->published.empty() ? S->tail : S->published.front().begin;
@@ -201,14 +218,17 @@ static bool pick_slice(nq_server *S, double now, Slice *out) {
     if (space < 2) return false;  // a task and the terminator
     auto key = [&](const std::shared_ptr<Request> &r) {
         const double rem = static_cast<double>(r->tasks.size() - r->next) * r->task_work;
-        return rem / 1e9 - 2.0 * (now - r->t_submit);  // ~1 ms per 1e9 nodes; 2x aging
+        // ~1 ms per 1e9 nodes; a request that has waited longer than AGING_AFTER_MS gains 2 ms
+        // of priority per ms, so that a stream of small requests cannot starve a large one.
+        return rem / 1e9 - 2.0 * std::max(0.0, now - r->t_submit - AGING_AFTER_MS);
     };
     std::shared_ptr<Request> best;
     for (auto &r : S->pending)
         if (!best || key(r) < key(best)) best = r;
     const uint64_t left = best->tasks.size() - best->next;
     uint64_t take = static_cast<uint64_t>(std::max(1.0, S->cfg.batch_work / best->task_work));
-    take = std::min<uint64_t>({take, left, space - 1, SLICE_MAX_TASKS});
+    const int weight = (!best->tasks.empty() && (best->tasks[0].w & (1u << 5))) ? 2 : 1;
+    take = std::min<uint64_t>({take, left, space - 1, slice_cap(best->free_rows, weight)});
     out->req = best;
     out->begin = S->tail;
     out->end = S->tail + take + 1;
@@ -229,8 +249,7 @@ static void write_slice(nq_server *S, const Slice &sl, size_t first) {
         u.w = tag | (u.w & 63u);
         S->ring[(sl.begin + j) & mask] = u;
     }
-    S->ring[(sl.end - 1) & mask] = make_uint4(static_cast<uint32_t>(sl.ntasks + 1), static_cast<uint32_t>(sl.seq),
-                                              static_cast<uint32_t>(sl.seq >> 32), tag | SERVE_TAG_TERM);
+    S->ring[(sl.end - 1) & mask] = make_uint4(static_cast<uint32_t>(sl.ntasks), 0, 0, tag | SERVE_TAG_TERM);
 }
 
 // The largest claim end the latest instance's waves have recorded: a lower bound of its claim
@@ -296,10 +315,8 @@ static hipError_t manage_instance(nq_server *S, double now) {
     a.ctl = S->ctl;
     a.done = S->done;
     a.claimed = S->claimed;
-    a.finished = S->finished;
-    a.count = S->count;
-    a.expect = S->expect;
-    a.seq = S->seq;
+    a.acc = S->acc;
+    a.gen = S->gen;
     a.error = S->error;
     a.orphan_ticks = static_cast<unsigned long long>(ORPHAN_S * 1000.0 * S->wall_khz);
     a.epoch = ++S->epoch;
@@ -326,18 +343,16 @@ static hipError_t manage_instance(nq_server *S, double now) {
 // slice is counted (lock held).
 static bool collect(nq_server *S, double now) {
     bool any = false;
-    for (size_t s = 0; s < S->slot_slice.size(); ++s) {
-        Slice *sl = S->slot_slice[s];
-        if (!sl) continue;
-        if (__atomic_load_n(&S->done[s].seq, __ATOMIC_ACQUIRE) != sl->seq) continue;
-        Request &r = *sl->req;
+    for (Slice &sl : S->published) {
+        if (sl.counted) continue;
+        const int s = sl.slot;
+        if (__atomic_load_n(&S->done[s].gen, __ATOMIC_ACQUIRE) != sl.gen) continue;
+        Request &r = *sl.req;
         r.count += __atomic_load_n(&S->done[s].count, __ATOMIC_RELAXED);
         r.outstanding--;
-        sl->counted = true;
-        if (!S->trace_path.empty())
-            S->trace.push_back({now - S->t_open, 'C', s, sl->seq, sl->end - sl->begin, 0});
-        S->slot_slice[s] = nullptr;
-        S->free_slots.push_back(static_cast<int>(s));
+        sl.counted = true;
+        if (!S->trace_path.empty()) S->trace.push_back({now - S->t_open, 'C', static_cast<uint64_t>(s), sl.seq, sl.ntasks, 0});
+        S->free_slots.push_back(s);
         S->inflight--;
         if (r.outstanding == 0 && r.next >= r.tasks.size() && !r.done) {
             r.done = true;
@@ -371,6 +386,7 @@ static void scheduler(nq_server *S) {
             sl.slot = S->free_slots.back();
             S->free_slots.pop_back();
             sl.seq = S->next_seq++;
+            sl.gen = ++S->slot_gen[sl.slot];
             const size_t first = r.next;
             r.next += sl.ntasks;
             r.outstanding++;
@@ -383,9 +399,8 @@ static void scheduler(nq_server *S) {
             write_slice(S, sl, first);
             lk.lock();
             if (!S->trace_path.empty())
-                S->trace.push_back({t_w - S->t_open, 'P', sl.begin, sl.end, S->claimed_seen, now_ms() - t_w});
+                S->trace.push_back({t_w - S->t_open, 'P', sl.begin, sl.end, sl.req->ticket, now_ms() - t_w});
             S->published.push_back(sl);
-            S->slot_slice[sl.slot] = &S->published.back();
             S->inflight++;
             S->tail = sl.end;
             __atomic_store_n(&S->ctl->tail[S->parity], S->tail, __ATOMIC_RELEASE);
@@ -438,7 +453,7 @@ int nq_server_config_default(nq_server_config *cfg) {
     cfg->depth_target = 12;
     cfg->block = 128;
     cfg->chunk = SERVE_BUF;
-    cfg->batch_work = 1e10;
+    cfg->batch_work = 1e9;
     return NQ_OK;
 }
 
@@ -473,10 +488,8 @@ int nq_server_open(const nq_server_config *cfg_in, nq_server **out) {
     if (e == hipSuccess) e = hipDeviceGetAttribute(&S->wall_khz, hipDeviceAttributeWallClockRate, S->device);
     if (e == hipSuccess) e = hipMalloc(&S->head, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&S->error, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&S->finished, nslots * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&S->count, nslots * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&S->expect, nslots * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&S->seq, nslots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&S->acc, nslots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&S->gen, nslots * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc(&S->ring, S->ring_cap * sizeof(uint4), hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostMalloc(&S->ctl, sizeof(ServeCtl), hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostMalloc(&S->done, nslots * sizeof(ServeDone), hipHostMallocCoherent);
@@ -485,9 +498,8 @@ int nq_server_open(const nq_server_config *cfg_in, nq_server **out) {
     if (e == hipSuccess) {
         // Zero the error word and slot counters with kernels (no memset).
         e = launch_zero(S->error, 1, S->stream);
-        if (e == hipSuccess) e = launch_zero(S->finished, nslots, S->stream);
-        if (e == hipSuccess) e = launch_zero(S->count, nslots, S->stream);
-        if (e == hipSuccess) e = launch_zero(S->expect, nslots, S->stream);
+        if (e == hipSuccess) e = launch_zero(S->acc, nslots, S->stream);
+        if (e == hipSuccess) e = launch_zero(S->gen, nslots, S->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
     }
     if (e == hipSuccess) {
@@ -509,8 +521,8 @@ int nq_server_open(const nq_server_config *cfg_in, nq_server **out) {
     memset(S->ctl, 0, sizeof(ServeCtl));
     memset(S->done, 0, nslots * sizeof(ServeDone));
     memset(S->claimed, 0, S->waves * sizeof(unsigned long long));
-    S->slot_slice.assign(nslots, nullptr);
     for (int s = static_cast<int>(nslots) - 1; s >= 0; --s) S->free_slots.push_back(s);
+    S->slot_gen.assign(nslots, 0);
     S->t_open = now_ms();
     if (const char *tp = getenv("NQ_SERVE_TRACE")) S->trace_path = tp;
     S->thr = std::thread(scheduler, S);
@@ -524,8 +536,22 @@ int nq_server_submit(nq_server *S, const nq_instance *inst, uint64_t *ticket) {
     if (!encode_instance(*inst, &c, &l, &r)) return fail(NQ_ERR_ARG, "invalid N/k or attacking prefix");
     auto req = std::make_shared<Request>();
     req->N = inst->N;
-    const int d0 = append_instance_tasks(inst->N, inst->k, c, l, r, S->cfg.depth_target, 0, req->tasks);
+    // Tasks keep at most depth_target free rows; a small request is split further, until it has
+    // SMALL_REQUEST_TASKS tasks or its tasks keep MIN_FREE_ROWS, so that many lanes share it
+    // (one lane alone would count a 12-row subtree about a thousand times slower than the device).
+    // (Never past the admission budget.)
+    int depth = S->cfg.depth_target;
+    int d0 = append_instance_tasks(inst->N, inst->k, c, l, r, depth, 0, req->tasks);
+    while (req->tasks.size() < SMALL_REQUEST_TASKS && inst->N - d0 > MIN_FREE_ROWS && depth > MIN_FREE_ROWS) {
+        TaskVec finer;
+        const int d1 = append_instance_tasks(inst->N, inst->k, c, l, r, depth - 1, 0, finer);
+        if (finer.size() > S->cfg.max_pending_tasks) break;
+        req->tasks.swap(finer);
+        d0 = d1;
+        --depth;
+    }
     req->task_work = est_task_nodes(inst->N - d0);
+    req->free_rows = std::max(0, inst->N - d0);
     // A request that can never fit the admission budget is refused for good (NQ_ERR_NOMEM), unlike
     // a full queue (NQ_ERR_BUSY), which a retry may pass.  (The ring takes it slice by slice.)
     if (req->tasks.size() > S->cfg.max_pending_tasks)
@@ -536,6 +562,9 @@ int nq_server_submit(nq_server *S, const nq_instance *inst, uint64_t *ticket) {
         return fail(NQ_ERR_BUSY, "pending task budget exhausted");
     req->ticket = S->next_ticket++;
     req->t_submit = now_ms();
+    if (!S->trace_path.empty())
+        S->trace.push_back({req->t_submit - S->t_open, 'A', req->ticket, req->tasks.size(),
+                            static_cast<uint64_t>(req->N), req->task_work});
     S->all[req->ticket] = req;
     if (req->tasks.empty()) {
         // No legal continuation: count 0, done at once.

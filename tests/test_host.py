@@ -340,3 +340,63 @@ def test_tuner_cache_parsed_by_the_library_reader(tmp_path):
     assert look("gfx950:a", 16)[0] == 0 and look("gfx950:a", 15)[0] == 0  # malformed, split > 4
     assert look("gfx942:b", 17)[:4] == [1, 17, 5, 1] and look("gfx950:a", 18)[0] == 0
     assert look("gfx950:a", 17, path="")[0] == 0
+
+
+def _expand_plain(N, c, l, r, rows):
+    """Every placement `rows` rows below (c, l, r), depth-first, lowest column first."""
+    last = (1 << N) - 1
+    out = []
+
+    def rec(c, l, r, k):
+        if k == 0:
+            out.append((c, l & 0xFFFFFFFF, r))
+            return
+        v = last & ~(c | l | r)
+        while v:
+            p = v & -v
+            v ^= p
+            rec(c | p, ((l | p) << 1) & 0xFFFFFFFF, (r | p) >> 1, k - 1)
+
+    rec(c, l, r, rows)
+    return np.array(out, dtype=np.uint32).reshape(-1, 3)
+
+
+def test_instance_expansion_equals_the_oracle(tmp_path):
+    """The 16-byte task lists of the batching and serving paths (append_instance_tasks: the
+    threaded two-pass expander of n_queens_amd/csrc/nq_prefix.cpp), built on the host with a small
+    driver: a whole board is the oracle's mirror-symmetric task list (tests also the threaded
+    path, 7 rows) with weight bit and tag; a prefixed instance is every placement below it in
+    depth-first order (6 rows: threaded as well)."""
+    drv = tmp_path / "drv.cpp"
+    drv.write_text('#include <cstdio>\n#include <cstdlib>\n#include "nq_tasks.h"\n'
+                   "int main(int argc, char **v) {\n"
+                   " int N = atoi(v[1]), dt = atoi(v[2]), k = argc - 4; uint32_t c = 0, l = 0, r = 0;\n"
+                   " for (int i = 0; i < k; ++i) { uint32_t p = 1u << atoi(v[4 + i]); c |= p; l = (l | p) << 1;"
+                   " r = (r | p) >> 1; }\n"
+                   " nq::TaskVec t; int d = nq::append_instance_tasks(N, k, c, l, r, dt, 5, t);\n"
+                   " FILE *f = fopen(v[3], \"wb\"); fwrite(t.data(), 16, t.size(), f); fclose(f);\n"
+                   ' printf("%d\\n", d); }\n')
+    exe = tmp_path / "drv"
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I",
+                           os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "n_queens_amd", "csrc"), str(drv),
+                           "-L", os.path.join(ROOT, "n_queens_amd"), "-lnq_hip",
+                           "-Wl,-rpath," + os.path.join(ROOT, "n_queens_amd"), "-o", str(exe)])
+
+    def run(N, dt, cols=()):
+        out = tmp_path / "t.bin"
+        d = int(subprocess.check_output([str(exe), str(N), str(dt), str(out), *map(str, cols)], env=NO_GPU))
+        return d, np.fromfile(out, dtype=np.uint32).reshape(-1, 4)
+
+    for N, dt in ((5, 12), (8, 5), (12, 8), (17, 10)):
+        rows = max(2, N - dt)
+        d, t = run(N, dt)
+        want = oracle.gen_tasks(N, rows)
+        assert d == rows and np.array_equal(t[:, :3], want), (N, dt)
+        assert np.all(t[:, 3] == (5 << 6) | (1 << 5) | N)
+    for N, dt, cols in ((10, 6, [0, 2]), (16, 8, [1, 3]), (13, 20, [4])):
+        enc = oracle.encode_prefix(N, cols)
+        s = max(0, N - len(cols) - dt)
+        d, t = run(N, dt, cols)
+        assert d == len(cols) + s
+        assert np.array_equal(t[:, :3], _expand_plain(N, *enc, s)), (N, cols)
+        assert np.all(t[:, 3] == (5 << 6) | N)
