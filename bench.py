@@ -36,7 +36,7 @@ LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
 # HBM bytes per N=21 rows-6 launch of the DFS kernel (split_kernel, 2 rows walked in registers),
 # from two rocprofv3 --pmc passes scaled by a 256 MiB copy (tests/probes/copy_probe: FETCH_SIZE
 # x2.000, WRITE_SIZE x1.000): profiles/split_kernel_n21_rows6_traffic.md.
-TRAFFIC_N21_ROWS6 = (2 * 62428 + 2975) * 1024
+TRAFFIC_N21_ROWS6 = (2 * 62422 + 2975) * 1024
 
 
 def log(msg: str) -> None:
