@@ -154,10 +154,11 @@ int nq_solve(int N, int rows, const nq_config *cfg, uint64_t *total, nq_stats *s
  * reference's n_queens(N, cur, left, right, sum) (src/n_queens.cpp:8-21) returns for the
  * prefix masks — with no symmetry fold.  k = 0 is the empty board.
  *
- * A batch plan expands every instance on the host into tasks (a few more rows, chosen so that
- * no task needs more than the stack depth), orders them largest-first, uploads them once, and
- * captures "zero counts, reset cursor, persistent batched DFS" into a hipGraph; each run
- * replays the graph on the plan's own stream.  Plans on different streams run concurrently. */
+ * A batch plan validates every instance on the host, orders their seeds largest-first and
+ * uploads them; the device expands them (a few more rows, until no task needs more than the
+ * stack depth, in the order a host depth-first expansion gives), and "zero counts, reset
+ * cursor, persistent batched DFS" is captured into a hipGraph; each run replays the graph on
+ * the plan's own stream.  Plans on different streams run concurrently. */
 typedef struct nq_instance {
     int N;
     int k;
