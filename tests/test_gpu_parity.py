@@ -314,3 +314,16 @@ def test_n21_node_count_fixture(published):
     r = nq.count(N, t, nodes=True)
     assert r.nodes == published["dfs_nodes"]["21,6"]
     assert 2 * r.sum == published["solutions"][N]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,rows", [(16, 4), (18, 5), (20, 6)])
+def test_solve_with_its_own_reported_config(N, rows, published, monkeypatch):
+    """nq_solve given back what nq_solve_config reports (stack depth for the depth-`rows` tasks,
+    the automatic split) counts exactly, as with no configuration (built-in rule: the tuner's
+    cache is switched off)."""
+    monkeypatch.setenv("NQ_TUNE_CACHE", "")
+    cfg = nq.solve_config(N, rows)
+    assert cfg.split > 0 and cfg.stack_depth == N - 1 - rows
+    assert nq.solve(N, rows, cfg)[0] == published["solutions"][N]
+    assert nq.solve(N, rows)[0] == published["solutions"][N]
