@@ -207,6 +207,7 @@ def serving(nq, log, n_req: int = 400, util: float = 0.7) -> dict:
            "slowdown_p50": float(np.percentile(slow, 50)), "slowdown_p99": float(np.percentile(slow, 99)),
            "whole_boards": pct(lat[whole]), "prefixed": pct(lat[~whole]),
            "alone_total_s": sum(alone_ms) / 1e3,
+           "alone_p50_ms": float(np.percentile(alone_ms, 50)), "alone_p99_ms": float(np.percentile(alone_ms, 99)),
            "batches": int(st["batches"]), "counts_identical_alone_vs_mix": same,
            "counts_match_oracle_fixture": oracle_ok and served == fx["counts"], "span_s": span,
            "busy_retries": busy[0], "busy_retries_capacity_probe": busy_burst,
