@@ -159,3 +159,31 @@ def test_served_poisson_set_vs_oracle_fixture():
                 raise AssertionError("request refused as busy for 60 s")
         got = [srv.wait(t, timeout_ms=120000)[0] for t in tickets]
     assert got == fx["counts"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg", [dict(chunk=1), dict(chunk=64, waves_per_cu=4), dict(depth_target=8),
+                                 dict(depth_target=14, block=256), dict(slot_tasks=1000, slot_requests=3),
+                                 dict(batch_work=1e6)])
+def test_server_configurations_exact(cfg):
+    """Served counts equal the alone counts across the server's configuration space: single-ticket
+    and full-buffer claims, low occupancy, shallow and deep tasks, a ring of 1024 entries with 3
+    slots, tiny slices.  (Shallow tasks (8 free rows) and the small ring get boards up to N=16 only:
+    a whole N=19 board would exceed the default task budget, or pass a 1024-entry ring in
+    thousands of 3-slice rounds.)"""
+    reqs = workloads.poisson_requests(40, lam=1000.0, seed=23)
+    if cfg.get("depth_target", 12) < 12 or "slot_tasks" in cfg:
+        reqs = [q for q in reqs if q[1] <= 16]
+    alone = [nq.instance_count(N, cols) for _, N, cols in reqs]
+    with nq.Server(**cfg) as srv:
+        tickets = []
+        for _, N, cols in reqs:
+            for _ in range(60000):
+                try:
+                    tickets.append(srv.submit(N, cols))
+                    break
+                except nq.NQError as e:
+                    assert e.status == nq.NQ_ERR_BUSY
+                    time.sleep(0.001)
+        got = [srv.wait(t, timeout_ms=60000)[0] for t in tickets]
+    assert got == alone
