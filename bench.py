@@ -8,7 +8,8 @@ resident in HBM.  The value is DFS nodes per second over the timed steps (a node
 placed at depths rows+1..N-1).  Also reported:
   * n21_wall_s: the whole `n_queens 21 6` CLI call (generation, device init, copies, kernel) —
     the reference's own measure (src/main.cpp:49-54; 21.242 s on one RTX 5090, README.md:24);
-  * roofline: integer-VALU bound of the DFS kernel (11 int32 ops per node);
+  * roofline: integer-VALU bound of the DFS kernel (11 int32 ops per node), with the measured
+    binding resource (the LDS push write) and the kernel's CU cycles per wave-iteration;
   * cpu_baseline: the reference's own OpenMP path (oracle/_ref, parallel_n_queens, which fixes
     32 threads) timed on this host on a bounded sample, and cpu_baseline_port: the oracle's
     restatement on every usable CPU of this host (one thread per CPU), same sample;
@@ -37,6 +38,10 @@ LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
 # from two rocprofv3 --pmc passes scaled by a 256 MiB copy (tests/probes/copy_probe: FETCH_SIZE
 # x2.000, WRITE_SIZE x1.000): profiles/split_kernel_n21_rows6_traffic.md.
 TRAFFIC_N21_ROWS6 = (2 * 62422 + 2975) * 1024
+# CU cycles per wave-iteration of the DFS loop's instruction stream at 12 waves/CU on fixed,
+# conflict-free LDS slots (tests/probes/loop_probe, mode 0): the ceiling its LDS traffic sets
+# (profiles/loop_probe_gfx950.md).
+LOOP_PROBE_CU_CYCLES = 19.35
 
 
 def log(msg: str) -> None:
@@ -365,7 +370,14 @@ def main() -> int:
                      "frac": achieved / peak,
                      "traffic": TRAFFIC_N21_ROWS6 if (N, rows) == (21, 6) else None,
                      "basis": f"{OPS_PER_NODE} int32 ops per DFS node x {nodes} nodes / kernel time; peak = "
-                              f"{info['cus']} CUs x 64 lanes x 2.4 GHz"},
+                              f"{info['cus']} CUs x 64 lanes x 2.4 GHz",
+                     # What binds the loop, measured apart from the algorithmic roofline: one
+                     # iteration per node and per completed board; CU cycles per wave-iteration
+                     # against the same instruction stream on fixed LDS slots.
+                     "binding": "LDS: one ds_write_b128 push and one ds_read_b128 pop per iteration",
+                     "cu_cycles_per_wave_iteration": (ms_per_step / 1e3) * PEAK_CLOCK_HZ * info["cus"]
+                                                     / ((nodes + sum0) / 64.0),
+                     "probe_cu_cycles_per_wave_iteration": LOOP_PROBE_CU_CYCLES},
     }
 
     if not args.no_wall:
