@@ -42,9 +42,10 @@ namespace nq {
 
 namespace {
 
-// Hot-loop trips (16 iterations each) between folds of the 32-bit per-lane counters: at most
-// 16 * 2^27 = 2^31 increments between folds.
-constexpr uint32_t FLUSH_EVERY = 1u << 27;
+// Hot-loop trips (16 iterations each) between folds of the 32-bit per-lane counters: a lane
+// adds at most 2 per iteration (the node counter: a placement and a forced move), so at most
+// 2 * 16 * 2^26 = 2^31 between folds.
+constexpr uint32_t FLUSH_EVERY = 1u << 26;
 // Trips between folds of the wave-uniform 32-bit counter (64 lanes add to it).
 constexpr uint32_t FLUSH_EVERY_WAVE = 1u << 21;
 // split_kernel's guided claims: a wave takes 1 / (GUIDED_SHARES x waves) of the tasks left.
@@ -112,9 +113,19 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 //   push {c,l,r,v} if v != 0                    (masked, or unconditional: see below)
 //   c |= p; l = (l|p) << 1; r = (r|p) >> 1      (l & p == 0: one v_add_lshl_u32)
 //   v = ~(l | r | c)                            (v_bitop3 table 0x01)
+//   if (popcount(v) == 1) {                     forced move: the child's only candidate is
+//       c |= v; l = (l|v) << 1; r = (r|v) >> 1  placed at once; that node keeps nothing to
+//       v = ~(l | r | c) }                      come back to, so it needs no push
 //   if (c == ~0) cnt += 1                       (compare + add with carry, or a scalar
 //                                                popcount of the compare mask per wave)
 //   if (v == 0) { sp = max(sp - stride, 0); {c,l,r,v} = lds[sp] }
+//
+// The forced move places a second queen in 37 % of iterations (the share of children with
+// exactly one candidate) for seven VALU and no LDS access, so one push and one pop serve 1.37
+// placements on average: the 16-byte push write bounds the plain iteration, and the VALU has
+// room.  A lane completes at most one board per iteration (a completed board's parent had one
+// candidate left and its child has none, so the board is never followed by a forced move), so
+// one count after the forced move counts both placements.
 //
 // The pop needs no stack-empty test.  LDS address 0 holds a dead frame DEAD_FRAME and every
 // lane's level 0 lies less than one stride above it, so the saturating subtract (v_sub_u32
@@ -151,16 +162,31 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
     "v_mad_u32_u24 %[sp], %[t], %[stride], %[sp]\n\t"
 
 // The child's right diagonal is started first so that no VALU waits on the one before it.
-// LANE_CNT adds the completed board (vcc) to the lane's counter; WAVE_CNT adds the popcount of
-// vcc to a wave-uniform SGPR counter and sits in the shadow of the pop's LDS read.
-#define NQ_CHILD_POP(LANE_CNT, WAVE_CNT)                                                      \
+#define NQ_CHILD                                                                              \
     "v_or_b32 v22, v22, %[p]\n\t"                                                             \
     "v_or_b32 v20, v20, %[p]\n\t"                                                             \
     "v_lshrrev_b32 v22, 1, v22\n\t"                                                           \
     "v_add_lshl_u32 v21, v21, %[p], 1\n\t"                                                    \
+    "v_bitop3_b32 v23, v21, v22, v20 bitop3:1\n\t"
+
+// The forced move, for the lanes whose child has exactly one candidate (v_cmpx on its
+// popcount): the candidate is v itself.  NODES_FORCED counts the extra expansion.
+#define NQ_FORCED(NODES_FORCED)                                                               \
+    "v_bcnt_u32_b32 %[t], v23, 0\n\t"                                                         \
+    "v_cmpx_eq_u32_e64 %[m1], 1, %[t]\n\t" NODES_FORCED                                       \
+    "v_or_b32 v22, v22, v23\n\t"                                                              \
+    "v_or_b32 v20, v20, v23\n\t"                                                              \
+    "v_lshrrev_b32 v22, 1, v22\n\t"                                                           \
+    "v_add_lshl_u32 v21, v21, v23, 1\n\t"                                                     \
+    "v_bitop3_b32 v23, v21, v22, v20 bitop3:1\n\t"                                            \
+    "s_mov_b64 exec, %[live]\n\t"
+
+// Count, then pop.  A completed board has no candidates, so its lane pops: LANE_CNT adds the
+// compare (vcc) to the lane's counter inside the pop's exec region.  WAVE_CNT adds the
+// popcount of vcc to a wave-uniform SGPR counter in the shadow of the pop's LDS read.
+#define NQ_COUNT_POP(LANE_CNT, WAVE_CNT)                                                      \
     "v_cmp_eq_u32 vcc, -1, v20\n\t"                                                           \
-    "v_bitop3_b32 v23, v21, v22, v20 bitop3:1\n\t" LANE_CNT                                   \
-    "v_cmpx_eq_u32_e64 %[m1], 0, v23\n\t"                                                     \
+    "v_cmpx_eq_u32_e64 %[m1], 0, v23\n\t" LANE_CNT                                            \
     "v_sub_u32_e64 %[sp], %[sp], %[stride] clamp\n\t"                                         \
     "ds_read_b128 v[20:23], %[sp]\n\t"                                                        \
     "s_mov_b64 exec, %[live]\n\t" WAVE_CNT                                                    \
@@ -171,23 +197,23 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
     "s_bcnt1_i32_b64 %[bc], vcc\n\t"                                                          \
     "s_add_u32 %[wcnt], %[wcnt], %[bc]\n\t"
 
-#define NQ_ITER(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
-    NODES_INSN NQ_EXTRACT PUSH NQ_CHILD_POP(LANE_CNT, WAVE_CNT)
-#define NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
-    NQ_ITER(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)
+#define NQ_ITER(PUSH, NODES_INSN, NODES_FORCED, LANE_CNT, WAVE_CNT)                           \
+    NODES_INSN NQ_EXTRACT PUSH NQ_CHILD NQ_FORCED(NODES_FORCED) NQ_COUNT_POP(LANE_CNT, WAVE_CNT)
+#define NQ_ITER2(PUSH, NI, NF, LANE_CNT, WAVE_CNT) \
+    NQ_ITER(PUSH, NI, NF, LANE_CNT, WAVE_CNT) NQ_ITER(PUSH, NI, NF, LANE_CNT, WAVE_CNT)
 
 // The whole hot loop: 16 iterations per trip, leaving after a trip that ends with some lane
 // on the dead frame (sp == 0), or after [iters] trips (the caller folds the 32-bit counters
 // then).  [nd] counts expansions (node-counting variant only).
-#define NQ_HOT_ASM(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)                                      \
+#define NQ_HOT_ASM(PUSH, NI, NF, LANE_CNT, WAVE_CNT)                                      \
     __asm__ volatile(                                                                         \
         "s_mov_b64 %[save], exec\n\t"                                                         \
         "s_mov_b64 exec, %[live]\n"                                                           \
-        "nq_hot_%=:\n\t" NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)                        \
-        NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
-        NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
-        NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT) \
-        NQ_ITER2(PUSH, NODES_INSN, LANE_CNT, WAVE_CNT)                                         \
+        "nq_hot_%=:\n\t" NQ_ITER2(PUSH, NI, NF, LANE_CNT, WAVE_CNT)                        \
+        NQ_ITER2(PUSH, NI, NF, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NI, NF, LANE_CNT, WAVE_CNT) \
+        NQ_ITER2(PUSH, NI, NF, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NI, NF, LANE_CNT, WAVE_CNT) \
+        NQ_ITER2(PUSH, NI, NF, LANE_CNT, WAVE_CNT) NQ_ITER2(PUSH, NI, NF, LANE_CNT, WAVE_CNT) \
+        NQ_ITER2(PUSH, NI, NF, LANE_CNT, WAVE_CNT)                                         \
         "v_cmp_eq_u32 %[m1], 0, %[sp]\n\t"                                                    \
         "s_cmp_lg_u64 %[m1], 0\n\t"                                                           \
         "s_cbranch_scc1 nq_exit_%=\n\t"                                                       \
@@ -202,15 +228,17 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
         : [live] "s"(live), [stride] "s"(stride)                                              \
         : "memory", "scc", "vcc")
 
-// Expansions of the lanes not on the dead frame (sp != 0 at the start of the iteration).
+// Expansions of the lanes not on the dead frame (sp != 0 at the start of the iteration), and
+// the forced move's expansion (inside its exec region).
 #define NQ_NODE_INC                                                                           \
     "v_min_u32 %[t], 1, %[sp]\n\t"                                                            \
     "v_add_u32 %[nd], %[t], %[nd]\n\t"
+#define NQ_NODE_FORCED "v_add_u32 %[nd], 1, %[nd]\n\t"
 
 // Runs the hot loop for the lanes in `live` until one of them finishes its task (or the fold
 // interval ends).  APUSH selects the unconditional push (the stack has a spare level).  WAVE
 // counts completed boards per wave into wcnt (SALU) instead of per lane into cnt; its fold
-// interval is 2^21 trips, at most 2^21 * 16 * 64 = 2^31 boards.
+// interval is 2^21 trips, at most 2^21 * 16 * 64 = 2^31 boards (one per lane and iteration).
 template <bool NODES, bool APUSH, bool WAVE>
 __device__ __forceinline__ void hot_loop(uint32_t &c, uint32_t &l, uint32_t &r, uint32_t &v, uint32_t &sp,
                                          uint32_t &cnt, uint32_t &wcnt, uint32_t &nd, uint64_t live,
@@ -222,19 +250,19 @@ __device__ __forceinline__ void hot_loop(uint32_t &c, uint32_t &l, uint32_t &r, 
     if (WAVE) {
         static_assert(!WAVE || !NODES, "the node-counting variant counts per lane");
         if (APUSH)
-            NQ_HOT_ASM(NQ_PUSH_ALWAYS, "", "", NQ_CNT_WAVE);
+            NQ_HOT_ASM(NQ_PUSH_ALWAYS, "", "", "", NQ_CNT_WAVE);
         else
-            NQ_HOT_ASM(NQ_PUSH_MASKED, "", "", NQ_CNT_WAVE);
+            NQ_HOT_ASM(NQ_PUSH_MASKED, "", "", "", NQ_CNT_WAVE);
     } else if (APUSH) {
         if (NODES)
-            NQ_HOT_ASM(NQ_PUSH_ALWAYS, NQ_NODE_INC, NQ_CNT_LANE, "");
+            NQ_HOT_ASM(NQ_PUSH_ALWAYS, NQ_NODE_INC, NQ_NODE_FORCED, NQ_CNT_LANE, "");
         else
-            NQ_HOT_ASM(NQ_PUSH_ALWAYS, "", NQ_CNT_LANE, "");
+            NQ_HOT_ASM(NQ_PUSH_ALWAYS, "", "", NQ_CNT_LANE, "");
     } else {
         if (NODES)
-            NQ_HOT_ASM(NQ_PUSH_MASKED, NQ_NODE_INC, NQ_CNT_LANE, "");
+            NQ_HOT_ASM(NQ_PUSH_MASKED, NQ_NODE_INC, NQ_NODE_FORCED, NQ_CNT_LANE, "");
         else
-            NQ_HOT_ASM(NQ_PUSH_MASKED, "", NQ_CNT_LANE, "");
+            NQ_HOT_ASM(NQ_PUSH_MASKED, "", "", NQ_CNT_LANE, "");
     }
 }
 

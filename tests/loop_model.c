@@ -8,13 +8,18 @@
  *   - every LDS write and read is the lane's own slot at a level below `levels`, or the dead
  *     frame at address 0 (and a write there leaves the dead frame unchanged);
  *   - every lane that ends a task reaches sp == 0 with v == 0;
- *   - every task's count equals a plain recursion (src/n_queens.cpp:8-21 semantics).
+ *   - every task's count equals a plain recursion (src/n_queens.cpp:8-21 semantics), and so
+ *     does its node count (expansions minus completed boards: the node-counting kernel's sum);
+ *   - a lane completes at most one board per iteration (the kernel counts once per iteration).
  * Both push forms: masked (write only when candidates remain) and unconditional (always write,
- * advance by min(v, 1), one spare level).
+ * advance by min(v, 1), one spare level).  Each iteration makes the forced move: a child with
+ * exactly one candidate has it placed at once, with no push.
  *
  * Build flags plant a mistake, to show that the checks catch it:
- *   -DPLANT_NO_CLAMP   the pop subtracts without saturating (reads below the stack);
- *   -DPLANT_FULL_DEAD  the dead frame has c == ~0 (parked lanes count boards).
+ *   -DPLANT_NO_CLAMP     the pop subtracts without saturating (reads below the stack);
+ *   -DPLANT_FULL_DEAD    the dead frame has c == ~0 (parked lanes count boards);
+ *   -DPLANT_FORCED_DEAD  the forced move is also made from a child with no candidate;
+ *   -DPLANT_COUNT_FIRST  boards are counted before the forced move (its boards are missed).
  * Exit status 0 when every case passes. */
 #include <stdint.h>
 #include <stdio.h>
@@ -40,6 +45,18 @@ static uint64_t completions(int N, uint32_t c, uint32_t l, uint32_t r) {
     for (uint32_t v = last & ~(c | l | r); v; v &= v - 1) {
         const uint32_t p = v & (0u - v);
         s += completions(N, c | p, (l | p) << 1, (r | p) >> 1);
+    }
+    return s;
+}
+
+/* DFS nodes below a prefix in the reference kernel's sense: one per queen placed at depths
+ * popcount(c)+1 .. N-1 (src/n_queens_cuda.cu:39-72). */
+static uint64_t nodes_below(int N, uint32_t c, uint32_t l, uint32_t r) {
+    const uint32_t last = (1u << N) - 1u;
+    uint64_t s = 0;
+    for (uint32_t v = last & ~(c | l | r); v; v &= v - 1) {
+        const uint32_t p = v & (0u - v);
+        if ((c | p) != last) s += 1 + nodes_below(N, c | p, (l | p) << 1, (r | p) >> 1);
     }
     return s;
 }
@@ -87,6 +104,8 @@ static int run(int N, int rows, int B, int apush) {
     frame *f = calloc((size_t)B, sizeof(frame));
     uint32_t *sp = calloc((size_t)B, sizeof(uint32_t));
     uint64_t *cnt = calloc((size_t)B, sizeof(uint64_t));
+    uint64_t *nd = calloc((size_t)B, sizeof(uint64_t)); /* expansions (NQ_NODE_INC, NQ_NODE_FORCED) */
+    uint64_t *gotn = calloc(ntasks ? ntasks : 1, sizeof(uint64_t));
     long *tid = malloc((size_t)B * sizeof(long));
     int *has = calloc((size_t)B, sizeof(int));
     uint64_t *got = calloc(ntasks ? ntasks : 1, sizeof(uint64_t));
@@ -101,8 +120,12 @@ static int run(int N, int rows, int B, int apush) {
         int any = 0;
         for (int i = 0; i < B; i++) { /* refill (dfs_kernel's claim loop) */
             if (has[i] && f[i].v == 0) {
-                if (tid[i] >= 0) got[tid[i]] = cnt[i];
+                if (tid[i] >= 0) {
+                    got[tid[i]] = cnt[i];
+                    gotn[tid[i]] = nd[i] - cnt[i];
+                }
                 cnt[i] = 0;
+                nd[i] = 0;
                 has[i] = 0;
                 tid[i] = -1;
                 while (cursor < ntasks) {
@@ -134,6 +157,7 @@ static int run(int N, int rows, int B, int apush) {
                 for (int i = 0; i < B; i++) {
                     if (!has[i]) continue;
                     frame *x = &f[i];
+                    if (sp[i] != 0) nd[i]++; /* NQ_NODE_INC */
                     const uint32_t p = x->v & (0u - x->v); /* NQ_EXTRACT */
                     x->v ^= p;
                     if (apush || x->v != 0) { /* NQ_PUSH_ALWAYS / NQ_PUSH_MASKED */
@@ -147,11 +171,33 @@ static int run(int N, int rows, int B, int apush) {
                         memcpy(lds + a, x, 16);
                         sp[i] += apush ? stride * (x->v != 0) : stride;
                     }
-                    x->c |= p; /* NQ_CHILD_POP */
+                    x->c |= p; /* NQ_CHILD */
                     x->l = (x->l + p) << 1;
                     x->r = (x->r | p) >> 1;
                     x->v = ~(x->l | x->r | x->c);
-                    if (x->c == 0xFFFFFFFFu) cnt[i]++;
+                    const int full_first = x->c == 0xFFFFFFFFu;
+#ifdef PLANT_COUNT_FIRST
+                    if (full_first) cnt[i]++;
+#endif
+#ifdef PLANT_FORCED_DEAD
+                    if ((x->v & (x->v - 1)) == 0) { /* NQ_FORCED with the empty set let in */
+#else
+                    if (__builtin_popcount(x->v) == 1) { /* NQ_FORCED: v_bcnt, v_cmpx_eq 1 */
+#endif
+                        const uint32_t q = x->v;
+                        nd[i]++; /* NQ_NODE_FORCED */
+                        x->c |= q;
+                        x->l = (x->l + q) << 1;
+                        x->r = (x->r | q) >> 1;
+                        x->v = ~(x->l | x->r | x->c);
+                        if (full_first && x->c == 0xFFFFFFFFu) {
+                            fprintf(stderr, "lane %d: two boards in one iteration\n", i);
+                            return 1;
+                        }
+                    }
+#ifndef PLANT_COUNT_FIRST
+                    if (x->c == 0xFFFFFFFFu) cnt[i]++; /* NQ_COUNT_POP */
+#endif
                     if (x->v == 0) {
 #ifdef PLANT_NO_CLAMP
                         sp[i] -= stride;
@@ -188,6 +234,17 @@ static int run(int N, int rows, int B, int apush) {
             rc = 1;
             break;
         }
+        /* Tasks counted at claim time (no DFS) have no nodes below them in the kernel's sense. */
+        const uint32_t c0 = tasks[3 * t], l0 = tasks[3 * t + 1], r0 = tasks[3 * t + 2];
+        const uint32_t d0 = (uint32_t)__builtin_popcount(c0);
+        const int direct = d0 + 1 >= (uint32_t)N || (last & ~(c0 | l0 | r0)) == 0;
+        const uint64_t wantn = direct ? 0 : nodes_below(N, c0, l0, r0);
+        if (gotn[t] != wantn) {
+            fprintf(stderr, "N=%d rows=%d B=%d apush=%d: task %zu nodes %llu, expected %llu\n", N, rows, B, apush, t,
+                    (unsigned long long)gotn[t], (unsigned long long)wantn);
+            rc = 1;
+            break;
+        }
         sum += got[t];
     }
     if (!rc) printf("N=%d rows=%d B=%d apush=%d tasks=%zu sum=%llu ok\n", N, rows, B, apush, ntasks,
@@ -196,6 +253,8 @@ static int run(int N, int rows, int B, int apush) {
     free(f);
     free(sp);
     free(cnt);
+    free(nd);
+    free(gotn);
     free(tid);
     free(has);
     free(got);

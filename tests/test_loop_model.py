@@ -1,8 +1,10 @@
 """The DFS loop's control logic on the CPU (tests/loop_model.c): the pop onto the dead frame at
 LDS address 0 stays within each lane's own stack slots, every finished lane parks there, and
-per-task counts are exact, for N=4..13, workgroups of 64..256 lanes and both push forms.  Two
-planted mistakes (a pop without saturation, a dead frame that looks like a full board) must
-make the model fail."""
+per-task counts and node counts are exact with the forced move (a child with one candidate has
+it placed in the same iteration), for N=4..13, workgroups of 64..256 lanes and both push forms.
+Four planted mistakes (a pop without saturation, a dead frame that looks like a full board, a
+forced move from a child with no candidate, a count taken before the forced move) must make the
+model fail."""
 import os
 import shutil
 import subprocess
@@ -34,7 +36,9 @@ def test_dead_frame_loop_in_bounds_and_exact(tmp_path):
 
 
 @pytest.mark.parametrize("plant, message", [("PLANT_NO_CLAMP", "bad read"),
-                                            ("PLANT_FULL_DEAD", "expected")])
+                                            ("PLANT_FULL_DEAD", "expected"),
+                                            ("PLANT_FORCED_DEAD", "two boards in one iteration"),
+                                            ("PLANT_COUNT_FIRST", "expected")])
 def test_planted_mistakes_are_caught(tmp_path, plant, message):
     out = subprocess.run([_build(tmp_path, plant), "8"], capture_output=True, text=True, timeout=300)
     assert out.returncode != 0
