@@ -9,7 +9,7 @@ placed at depths rows+1..N-1).  Also reported:
   * n21_wall_s: the whole `n_queens 21 6` CLI call (generation, device init, copies, kernel) —
     the reference's own measure (src/main.cpp:49-54; 21.242 s on one RTX 5090, README.md:24);
   * roofline: integer-VALU bound of the DFS kernel (11 int32 ops per node), with the measured
-    binding resource (the LDS push write) and the kernel's CU cycles per wave-iteration;
+    binding resource and the kernel's CU cycles per queen placed by one wave;
   * cpu_baseline: the reference's own OpenMP path (oracle/_ref, parallel_n_queens, which fixes
     32 threads) timed on this host on a bounded sample, and cpu_baseline_port: the oracle's
     restatement on every usable CPU of this host (one thread per CPU), same sample;
@@ -37,11 +37,7 @@ LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
 # HBM bytes per N=21 rows-6 launch of the DFS kernel (split_kernel, 2 rows walked in registers),
 # from two rocprofv3 --pmc passes scaled by a 256 MiB copy (tests/probes/copy_probe: FETCH_SIZE
 # x2.000, WRITE_SIZE x1.000): profiles/split_kernel_n21_rows6_traffic.md.
-TRAFFIC_N21_ROWS6 = (2 * 62422 + 2975) * 1024
-# CU cycles per wave-iteration of the DFS loop's instruction stream at 12 waves/CU on fixed,
-# conflict-free LDS slots (tests/probes/loop_probe, mode 0): the ceiling its LDS traffic sets
-# (profiles/loop_probe_gfx950.md).
-LOOP_PROBE_CU_CYCLES = 19.35
+TRAFFIC_N21_ROWS6 = round((2 * 62471.6 + 2975.0) * 1024)
 
 
 def log(msg: str) -> None:
@@ -371,13 +367,14 @@ def main() -> int:
                      "traffic": TRAFFIC_N21_ROWS6 if (N, rows) == (21, 6) else None,
                      "basis": f"{OPS_PER_NODE} int32 ops per DFS node x {nodes} nodes / kernel time; peak = "
                               f"{info['cus']} CUs x 64 lanes x 2.4 GHz",
-                     # What binds the loop, measured apart from the algorithmic roofline: one
-                     # iteration per node and per completed board; CU cycles per wave-iteration
-                     # against the same instruction stream on fixed LDS slots.
-                     "binding": "LDS: one ds_write_b128 push and one ds_read_b128 pop per iteration",
-                     "cu_cycles_per_wave_iteration": (ms_per_step / 1e3) * PEAK_CLOCK_HZ * info["cus"]
-                                                     / ((nodes + sum0) / 64.0),
-                     "probe_cu_cycles_per_wave_iteration": LOOP_PROBE_CU_CYCLES},
+                     # What binds the loop, from the SQ counters of this launch
+                     # (profiles/split_kernel_n21_rows6_sq_counters.md), and the CU cycles one
+                     # wave spends per queen placed (nodes and completed boards).
+                     "binding": "VALU issue, 0.95 of the wave-instruction peak: 20 VALU, one ds_write_b128 "
+                                "push and one ds_read_b128 pop per iteration, 1.37 placements per iteration "
+                                "(forced moves)",
+                     "cu_cycles_per_wave_placement": (ms_per_step / 1e3) * PEAK_CLOCK_HZ * info["cus"]
+                                                     / ((nodes + sum0) / 64.0)},
     }
 
     if not args.no_wall:
