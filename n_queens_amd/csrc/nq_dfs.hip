@@ -6,8 +6,9 @@
 //
 // Design (one lane = one independent DFS, 64 lanes per wave):
 //  * The frame being expanded lives in VGPRs: c (occupied columns), l, r (diagonals), v
-//    (candidates not yet tried).  One loop iteration places exactly one queen (one DFS node)
-//    on every active lane; there is no per-iteration reload of the top frame.
+//    (candidates not yet tried).  One loop iteration places one queen on every active lane,
+//    and a second one where the first queen's child has exactly one candidate (the forced
+//    move: that child needs no push); there is no per-iteration reload of the top frame.
 //  * Before a frame is replaced by its child it is pushed to LDS only if it still has
 //    untried candidates (tail-call elimination); a lane whose frame runs empty pops.
 //    Frames are 16 B {c, l, r, v} at stack[level][threadIdx.x], behind a dead frame at LDS
@@ -27,9 +28,10 @@
 //  * A lane whose task is exhausted waits on the dead frame until the end of its 16-iteration
 //    trip, then claims the next task (lane-granular refill) from a global cursor; claims are
 //    aggregated per wave and handed out in chunks through a wave-private LDS range.
-//  * Per-lane counts are 32-bit (a lane adds at most 1 per iteration) and folded into 64-bit
-//    every 2^31 iterations and at task end; the 64-bit sums are reduced per wave with DPP
-//    shuffles, per workgroup through LDS, and added with one atomic per workgroup.
+//  * Per-lane counts are 32-bit (a lane adds at most one board and two expansions per
+//    iteration) and folded into 64-bit at most every 2^31 increments and at task end; the
+//    64-bit sums are reduced per wave with DPP shuffles, per workgroup through LDS, and added
+//    with one atomic per workgroup.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
