@@ -1,8 +1,9 @@
-// Timing probe for gfx950: the DFS hot loop's instruction mix (13 VALU, the exec switch of the pop,
-// the per-wave board count, one LDS write and one LDS read per iteration) at fixed LDS addresses
-// and a fixed trip count, with the LDS accesses changed one at a time, at 4, 8, 12 and 16 resident
-// waves per CU.  It locates the resource that bounds the DFS iteration: which LDS access, and at
-// what width, the loop can afford.
+// Timing probe for gfx950: the plain DFS hot loop's instruction mix (13 VALU, the exec switch of
+// the pop, the per-wave board count, one LDS write and one LDS read per iteration) at fixed LDS
+// addresses and a fixed trip count, with the LDS accesses changed one at a time, at 4, 8, 12 and
+// 16 resident waves per CU.  It locates the resource that bounds the DFS iteration: which LDS
+// access, and at what width, the loop can afford.  The last mode is the product's iteration with
+// the forced move (20 VALU per iteration for 1.37 placements).
 //
 // Every lane keeps one slot per level (lane-interleaved 16-B slots, no bank conflicts, the DFS
 // stack's pattern) and never moves its stack pointer: the pointer arithmetic of the loop is kept
@@ -86,6 +87,35 @@
     "s_add_u32 %[wcnt], %[wcnt], %[bc]\n\t"                                                       \
     "s_waitcnt lgkmcnt(0)\n\t"
 
+// The product's iteration with the forced move (nq_dfs.hip: NQ_EXTRACT, NQ_PUSH_ALWAYS, NQ_CHILD,
+// NQ_FORCED, NQ_COUNT_POP with the per-wave count): 20 VALU, one push and one pop.
+#define ITER_FORCED                                                                               \
+    "v_sub_u32 %[t], 0, v23\n\t"                                                                  \
+    "v_and_b32 %[p], v23, %[t]\n\t"                                                               \
+    "v_xor_b32 v23, v23, %[p]\n\t" W128                                                          \
+    "v_min_u32 %[t], 1, v23\n\t"                                                                  \
+    "v_mad_u32_u24 %[d], %[t], 0, %[d]\n\t"                                                       \
+    "v_or_b32 v22, v22, %[p]\n\t"                                                                 \
+    "v_or_b32 v20, v20, %[p]\n\t"                                                                 \
+    "v_lshrrev_b32 v22, 1, v22\n\t"                                                               \
+    "v_add_lshl_u32 v21, v21, %[p], 1\n\t"                                                        \
+    "v_bitop3_b32 v23, v21, v22, v20 bitop3:1\n\t"                                                \
+    "v_bcnt_u32_b32 %[t], v23, 0\n\t"                                                             \
+    "v_cmpx_eq_u32_e64 %[m1], 1, %[t]\n\t"                                                        \
+    "v_or_b32 v22, v22, v23\n\t"                                                                  \
+    "v_or_b32 v20, v20, v23\n\t"                                                                  \
+    "v_lshrrev_b32 v22, 1, v22\n\t"                                                               \
+    "v_add_lshl_u32 v21, v21, v23, 1\n\t"                                                         \
+    "v_bitop3_b32 v23, v21, v22, v20 bitop3:1\n\t"                                                \
+    "s_mov_b64 exec, %[live]\n\t"                                                                 \
+    "v_cmp_eq_u32 vcc, -1, v20\n\t"                                                               \
+    "v_cmpx_eq_u32_e64 %[m1], 0, v23\n\t"                                                         \
+    "v_sub_u32_e64 %[d], %[d], 0 clamp\n\t" R128                                                  \
+    "s_mov_b64 exec, %[live]\n\t"                                                                 \
+    "s_bcnt1_i32_b64 %[bc], vcc\n\t"                                                              \
+    "s_add_u32 %[wcnt], %[wcnt], %[bc]\n\t"                                                       \
+    "s_waitcnt lgkmcnt(0)\n\t"
+
 #define X4(B) B B B B
 #define X16(B) X4(B) X4(B) X4(B) X4(B)
 
@@ -101,13 +131,14 @@
                      : [sel] "v"(sel)                                                             \
                      : "memory", "scc", "vcc", "v24", "v25", "v26", "v27", "v28")
 
-constexpr int NMODES = 11;
+constexpr int NMODES = 12;
 const char *const MODE_NAMES[NMODES] = {
     "b128 write + b128 read (the DFS)", "b128 read only",         "b128 write only",
     "no LDS access",                    "b96 write + b128 read",  "b64 write + b128 read",
     "b32 write + b128 read",            "b128 write, 25% lanes",  "b128 write, all lanes, same exec switch",
     "b128 read issued before the b128 write",
-    "the DFS plus one VALU (14 VALU)"};
+    "the DFS plus one VALU (14 VALU)",
+    "the DFS with the forced move (20 VALU)"};
 
 template <int MODE>
 __global__ void __launch_bounds__(256) loop_probe(uint32_t trips, uint32_t *sink) {
@@ -131,6 +162,7 @@ __global__ void __launch_bounds__(256) loop_probe(uint32_t trips, uint32_t *sink
     if (MODE == 8) LOOP(ITER(WM100, R128));
     if (MODE == 9) LOOP(ITER_READ_FIRST);
     if (MODE == 10) LOOP(ITER(W128V, R128));
+    if (MODE == 11) LOOP(ITER_FORCED);
     sink[blockIdx.x * 256 + threadIdx.x] = c ^ l ^ r ^ v ^ wcnt;
 }
 
@@ -144,7 +176,7 @@ int main(int argc, char **argv) {
     const int cus = prop.multiProcessorCount;
     const kfn ks[NMODES] = {loop_probe<0>, loop_probe<1>, loop_probe<2>, loop_probe<3>, loop_probe<4>,
                             loop_probe<5>, loop_probe<6>, loop_probe<7>, loop_probe<8>, loop_probe<9>,
-                            loop_probe<10>};
+                            loop_probe<10>, loop_probe<11>};
     const int wpcs[4] = {4, 8, 12, 16};
     // LDS per 256-lane workgroup such that exactly waves/4 workgroups fit in 160 KiB.
     const size_t ldsz[4] = {96 * 1024, 60 * 1024, 44 * 1024, 34 * 1024};
