@@ -168,7 +168,7 @@ typedef struct nq_instance {
 typedef struct nq_batch_stats {
     uint64_t ntasks;     /* device tasks after expansion                                */
     uint64_t total;      /* sum of all instance counts                                  */
-    double expand_ms;    /* host validation + expansion (plan creation)                 */
+    double expand_ms;    /* plan creation: host validation and seeds, device expansion  */
     double kernel_ms;    /* device time of the replayed graph (HIP events)              */
     double total_ms;     /* whole call (nq_batch_count only)                            */
 } nq_batch_stats;
