@@ -118,13 +118,25 @@ def batched_mix(nq, log) -> dict:
     plan.close()
     same = [int(x) for x in counts] == per and [int(x) for x in c2] == per
     fx = instance_fixture("batch_mix")
-    oracle_ok = fx["digest"] == workloads.instances_digest(mix) and per == fx["counts"]
+    digest = workloads.instances_digest(mix)
+    oracle_ok = fx["digest"] == digest and per == fx["counts"]
+    # The mix's DFS node total (the oracle's, tests/golden/mix_nodes.json): the batched kernel's
+    # node rate, comparable with the whole-board kernel's nodes/s.
+    mix_nodes = None
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "mix_nodes.json")) as f:
+            mn = json.load(f)
+        if mn["digest"] == digest:
+            mix_nodes = int(mn["total"])
+    except (OSError, KeyError, ValueError):
+        pass
     out = {"instances": n, "per_launch_inst_per_s": n / t_per, "batch_inst_per_s": n / t_batch,
            "batch_replay_inst_per_s": n / t_replay, "speedup": t_per / t_batch,
            "speedup_replay": t_per / t_replay, "counts_identical": same,
            "counts_match_oracle_fixture": oracle_ok, "per_launch_s": t_per,
            "batch_s": t_batch, "batch_replay_s": t_replay, "batch_kernel_ms": st2["kernel_ms"],
-           "ntasks": int(st["ntasks"]), "solutions": int(sum(per)),
+           "ntasks": int(st["ntasks"]), "solutions": int(sum(per)), "nodes": mix_nodes,
+           "batch_kernel_nodes_per_s": mix_nodes / (st2["kernel_ms"] / 1e3) if mix_nodes else None,
            "workload": "512 instances, N uniform in [14,20], 2-6 fixed rows, seed 0x6E51"}
     log(f"batched mix: per-launch {t_per:.3f}s  batch {t_batch:.3f}s  replay {t_replay:.3f}s  "
         f"speedup {t_per / t_batch:.2f}x (replay {t_per / t_replay:.2f}x)  identical={same} oracle={oracle_ok}")

@@ -65,3 +65,64 @@ def test_fixture_tiny_instances_vs_brute_force(fixture):
                 assert c == n, (N, cols)
                 checked += 1
     assert checked >= 5
+
+
+MIX_NODES = os.path.join(ROOT, "tests", "golden", "mix_nodes.json")
+
+
+@pytest.fixture(scope="module")
+def mix_nodes():
+    with open(MIX_NODES) as f:
+        return json.load(f)
+
+
+def test_mix_nodes_belong_to_the_seeded_mix(mix_nodes):
+    """The node fixture (tests/golden/make_mix_nodes.py) is for the 512 seeded instances."""
+    mix = workloads.batch_mix()
+    assert mix_nodes["digest"] == workloads.instances_digest(mix)
+    assert len(mix_nodes["nodes"]) == len(mix) == 512
+    assert sum(mix_nodes["nodes"]) == mix_nodes["total"]
+
+
+def _nodes_by_sets(N, cols):
+    """DFS nodes below a prefix without bitmasks: queens placed row by row on columns and the
+    two diagonal sets, counted at depths k+1..N-1 (a full board is not a node)."""
+    used = set(cols)
+    d1 = {i + x for i, x in enumerate(cols)}
+    d2 = {i - x for i, x in enumerate(cols)}
+
+    def walk(row):
+        if row == N:
+            return 0
+        n = 0
+        for x in range(N):
+            if x in used or row + x in d1 or row - x in d2:
+                continue
+            if row < N - 1:
+                n += 1
+            used.add(x); d1.add(row + x); d2.add(row - x)
+            n += walk(row + 1)
+            used.discard(x); d1.discard(row + x); d2.discard(row - x)
+        return n
+
+    return walk(len(cols))
+
+
+def test_mix_nodes_small_instances_by_sets(mix_nodes):
+    """Instances with at most 9 free rows, counted with sets here (no bitmask code shared)."""
+    checked = 0
+    for (N, cols), n in zip(workloads.batch_mix(), mix_nodes["nodes"]):
+        if N - len(cols) <= 9:
+            assert n == _nodes_by_sets(N, cols), (N, cols)
+            checked += 1
+    assert checked >= 5
+
+
+def test_mix_nodes_medium_instances_vs_oracle(mix_nodes):
+    """Instances with at most 12 free rows, recounted by the oracle's node counter."""
+    checked = 0
+    for (N, cols), n in zip(workloads.batch_mix(), mix_nodes["nodes"]):
+        if N - len(cols) <= 12:
+            assert n == oracle.count_nodes(N, *oracle.encode_prefix(N, cols)), (N, cols)
+            checked += 1
+    assert checked >= 20
