@@ -125,9 +125,8 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 // The forced move places a second queen in 37 % of iterations (the share of children with
 // exactly one candidate) for seven VALU and no LDS access, so one push and one pop serve 1.37
 // placements on average: the 16-byte push write bounds the plain iteration, and the VALU has
-// room.  A lane completes at most one board per iteration (a completed board's parent had one
-// candidate left and its child has none, so the board is never followed by a forced move), so
-// one count after the forced move counts both placements.
+// room.  A lane completes at most one board per iteration (a completed board has no candidate,
+// so no forced move follows it), so one count after the forced move counts both placements.
 //
 // The pop needs no stack-empty test.  LDS address 0 holds a dead frame DEAD_FRAME and every
 // lane's level 0 lies less than one stride above it, so the saturating subtract (v_sub_u32
