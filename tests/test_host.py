@@ -400,3 +400,12 @@ def test_instance_expansion_equals_the_oracle(tmp_path):
         assert d == len(cols) + s
         assert np.array_equal(t[:, :3], _expand_plain(N, *enc, s)), (N, cols)
         assert np.all(t[:, 3] == (5 << 6) | N)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it says so and exits before touching a device."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr and p.stdout == ""
