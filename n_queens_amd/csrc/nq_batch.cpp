@@ -9,7 +9,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "nq.h"
@@ -64,6 +66,49 @@ int append_instance_tasks(int N, int k, uint32_t c, uint32_t l, uint32_t r, int 
     return k + s;
 }
 
+double est_nodes_per_task(int N, int k, uint32_t c, uint32_t l, uint32_t r, int split) {
+    constexpr int PROBES = 64;
+    const int m = N - k;  // free rows
+    if (m <= 0) return 0;
+    split = std::min(std::max(split, 0), m);
+    const uint32_t last = (1u << N) - 1u;
+    // xorshift64 (Marsaglia 2003) seeded from the prefix words (never zero)
+    uint64_t s = ((static_cast<uint64_t>(c) << 32 | l) ^ (static_cast<uint64_t>(r) << 7) ^ static_cast<uint64_t>(N)) |
+                 (1ull << 63);
+    auto rnd = [&s]() {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        return static_cast<uint32_t>(s >> 32);
+    };
+    // est[j]: estimated nodes j rows below the prefix, summed over the probes.
+    double est[33] = {0};
+    for (int p = 0; p < PROBES; ++p) {
+        uint32_t pc = c, pl = l, pr = r;
+        double w = 1;
+        for (int j = 0; j <= m; ++j) {
+            est[j] += w;
+            if (j == m) break;
+            uint32_t v = last & ~(pc | pl | pr);
+            const int b = __builtin_popcount(v);
+            if (b == 0) break;
+            w *= b;
+            // a uniform candidate: the i-th set bit, i = floor(rnd * b / 2^32)
+            for (uint32_t i = static_cast<uint32_t>((static_cast<uint64_t>(rnd()) * b) >> 32); i; --i) v &= v - 1;
+            const uint32_t q = v & (0u - v);
+            pc |= q;
+            pl = (pl | q) << 1;
+            pr = (pr | q) >> 1;
+        }
+    }
+    if (est[split] <= 0) return 0;
+    // DFS nodes of one task: placements below its root, up to the last row (a full board is
+    // not a node).
+    double below = 0;
+    for (int j = split + 1; j < m; ++j) below += est[j];
+    return below / est[split];
+}
+
 bool encode_instance(const nq_instance &in, uint32_t *c, uint32_t *l, uint32_t *r) {
     Prefix p;
     if (!encode(in, &p)) return false;
@@ -89,18 +134,28 @@ int build_batch_seeds(const nq_instance *inst, int n, int depth_target, std::vec
             return fail(NQ_ERR_ARG, buf);
         }
     }
-    // Order instances by remaining free rows after the split (then by N), descending: the
-    // biggest subtrees start first and small ones fill the tail.
+    // Order instances by estimated DFS nodes per task, descending: the longest tasks start
+    // first and the shortest fill the tail.  (Free rows alone are a poor proxy: a 12-row task
+    // of N=14 holds about 13 times the nodes of a 12-row task of N=20, and a lane that claims
+    // one of those last runs alone for milliseconds.)
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
-    auto rem_rows = [&](int i) {
-        const int N = inst[i].N, k = inst[i].k;
-        return (N - k) - split_rows(N, k, depth_target);
+    // About 15 us per instance of 20 free rows: large batches spread them over threads.
+    std::vector<double> est(n);
+    std::atomic<int> next_est{0};
+    auto estimate = [&] {
+        for (int i = next_est++; i < n; i = next_est++) {
+            const int N = inst[i].N, k = inst[i].k;
+            const int s = split_rows(N, k, depth_target);
+            est[i] = est_nodes_per_task(N, k, pre[i].c, pre[i].l, pre[i].r, k == 0 && N >= 4 ? std::max(2, s) : s);
+        }
     };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        const int ra = rem_rows(a), rb = rem_rows(b);
-        return ra != rb ? ra > rb : inst[a].N > inst[b].N;
-    });
+    const int threads = n >= 64 ? static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency()))) : 1;
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(estimate);
+    estimate();
+    for (auto &t : pool) t.join();
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return est[a] > est[b]; });
     seeds.clear();
     int frames = 1, lev = 0;
     for (int i : order) {

@@ -592,6 +592,12 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
         acc += x;
     };
 
+    // The lane's next task, claimed and loaded one task ahead: its load completes while the
+    // current task runs, so a refill does not wait for memory (tasks average about 1e4 nodes,
+    // and every refill stalls the whole wave).
+    uint4 ahead = make_uint4(0u, 0u, 0u, 0u);
+    bool have_ahead = false;
+
     for (;;) {
         if (has && v == 0) {
             if (own) add(inst, tsum * weight);
@@ -599,9 +605,17 @@ batch_kernel(const uint4 *__restrict__ tasks, uint64_t ntasks, uint32_t chunk, u
             own = false;
             has = false;
             for (;;) {
-                const uint64_t t = claim(wq[wave], &st->cursor, chunk);
-                if (t >= ntasks) break;
-                const uint4 w = tasks[t];
+                uint4 w;
+                if (have_ahead) {
+                    w = ahead;
+                } else {
+                    const uint64_t t = claim(wq[wave], &st->cursor, chunk);
+                    if (t >= ntasks) break;
+                    w = tasks[t];
+                }
+                const uint64_t t_next = claim(wq[wave], &st->cursor, chunk);
+                have_ahead = t_next < ntasks;
+                if (have_ahead) ahead = tasks[t_next];
                 const uint32_t N = w.w & 31u;
                 weight = 1u + ((w.w >> 5) & 1u);
                 inst = w.w >> 6;

@@ -52,4 +52,11 @@ void expand_tagged(int N, const ExpandPart *parts, int nparts, int rows, uint32_
 int append_instance_tasks(int N, int k, uint32_t c, uint32_t l, uint32_t r, int depth_target, uint32_t idx,
                           TaskVec &out);
 
+// Estimated DFS nodes per task of a prefix (board mask in c not set; k queens placed) whose
+// tasks lie `split` rows below it: Knuth's random-probe estimate of the node count at every
+// depth of the subtree (D. E. Knuth, "Estimating the efficiency of backtrack programs", Math.
+// Comp. 29 (1975)), from a fixed number of probes with a generator seeded by the prefix, so the
+// same prefix always gets the same estimate.  Orders work only; never affects a count.
+double est_nodes_per_task(int N, int k, uint32_t c, uint32_t l, uint32_t r, int split);
+
 }  // namespace nq
