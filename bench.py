@@ -232,6 +232,38 @@ def serving(nq, log, n_req: int = 400, util: float = 0.7) -> dict:
     return out
 
 
+def device_generation(nq, log) -> dict:
+    """The reference's task list generated on the device (nq_generate_device) into a resident
+    device buffer, for N=21 rows 6 and N=27 rows 7 (453,688,251 tasks, 5.4 GB): task counts
+    checked against the reference logs, best of 3 synchronous calls beside the reference's host
+    generation time (one thread, its 5090 log)."""
+    import torch
+
+    with open(os.path.join(ROOT, "tests", "golden", "published.json")) as f:
+        fx = json.load(f)
+    logged = {(n, r): k for n, r, k, _ in fx["task_counts"]}
+    out = {}
+    for N, rows in ((21, 6), (27, 7)):
+        n = nq.generate_device(N, rows)  # the count alone
+        buf = torch.empty(3 * n, dtype=torch.int32, device="cuda:0")
+        torch.cuda.synchronize()
+        nq.generate_device(N, rows, buf.data_ptr(), n)
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            got = nq.generate_device(N, rows, buf.data_ptr(), n)
+            best = min(best, time.perf_counter() - t0)
+        del buf
+        torch.cuda.empty_cache()
+        ref_ms = fx["generation_ms"][f"{N},{rows}"][0]
+        out[f"{N},{rows}"] = {"tasks": got, "matches_log": got == logged.get((N, rows)), "ms": best * 1e3,
+                              "reference_host_ms": ref_ms, "bytes_written": 12 * got,
+                              "write_GB_per_s": 12 * got / best / 1e9}
+        log(f"device generation N={N} rows={rows}: {got} tasks in {best * 1e3:.2f} ms "
+            f"(reference host {ref_ms} ms), log count {'ok' if got == logged.get((N, rows)) else 'DIFFERS'}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +278,8 @@ def main() -> int:
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--no-gen", action="store_true", help="skip the device task-generation figures")
+    ap.add_argument("--gen-only", action="store_true")
     ap.add_argument("--split", type=int, default=-1, help="device split rows (-1: nq_solve's choice)")
     ap.add_argument("--block", type=int, default=0)
     args = ap.parse_args()
@@ -268,6 +302,9 @@ def main() -> int:
         return 0
     if args.serve_only:
         print(json.dumps({"serving": serving(nq, log)}), flush=True)
+        return 0
+    if args.gen_only:
+        print(json.dumps({"device_generation": device_generation(nq, log)}), flush=True)
         return 0
     t0 = time.perf_counter()
     tasks = nq.generate(N, rows)
@@ -407,6 +444,8 @@ def main() -> int:
         out["batched_mix"] = batched_mix(nq, log)
     if not args.no_serve:
         out["serving"] = serving(nq, log)
+    if not args.no_gen:
+        out["device_generation"] = device_generation(nq, log)
 
     if not args.no_cpu:
         def nodes_fn(n, r):

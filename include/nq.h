@@ -117,6 +117,18 @@ int nq_config_resolve(int N, int d0, uint64_t ntasks, const nq_config *cfg, nq_c
  * returned in *ntasks.  Rows must be in 1..N. */
 int nq_generate(int N, int rows, uint32_t *tasks, uint64_t cap, uint64_t *ntasks);
 
+/* The same task list generated on the current device, straight into device memory: replaces the
+ * host generation and the host-to-device copy of cuda_n_queens (src/n_queens_cuda.cu:91-95,
+ * :156-160; partial_n_queens(_for_odd), src/n_queens.cpp:74-113).  The host places rows 0 and
+ * 1 (the reference's restricted rows: a few hundred tasks at most); every deeper row is expanded
+ * on the device, order-preserving (child count, scan, scatter), so d_tasks receives exactly
+ * nq_generate's words in the same order.  d_tasks is a device pointer (3 * cap uint32); with
+ * d_tasks == NULL or cap too small only *ntasks is set.  Intermediate rows are device memory the
+ * call allocates and frees; stream is a hipStream_t (NULL = default stream), synchronised before
+ * the call returns.  N 1..31, rows 1..N; NQ_ERR_NOMEM when device memory is short or a row
+ * above the last holds more than 2^31-1 tasks. */
+int nq_generate_device(int N, int rows, uint32_t *d_tasks, uint64_t cap, uint64_t *ntasks, void *stream);
+
 /* ---- counting --------------------------------------------------------------------------
  * Replaces the per-GPU region of cuda_n_queens (src/n_queens_cuda.cu:146-179: H2D copy,
  * kernel n_queens :9-82, D2H of partial_sum) with one device consuming the whole list.

@@ -32,10 +32,10 @@ NQ_ERR_BUSY = -6
 NQ_ERR_STATE = -7
 NQ_PUSH_AUTO, NQ_PUSH_MASKED, NQ_PUSH_ALWAYS = 0, 1, 2
 
-# Every symbol include/nq.h declares (checked by tests/test_abi.py).
+# Every symbol include/nq.h declares (checked by tests/test_host.py).
 EXPORTS = (
     "nq_device_count", "nq_last_error", "nq_set_device", "nq_device_info", "nq_config_default",
-    "nq_generate", "nq_count", "nq_count_device", "nq_solve", "nq_solve_config", "nq_config_resolve",
+    "nq_generate", "nq_generate_device", "nq_count", "nq_count_device", "nq_solve", "nq_solve_config", "nq_config_resolve",
     "nq_batch_plan_create", "nq_batch_plan_launch", "nq_batch_plan_wait", "nq_batch_plan_run",
     "nq_batch_plan_destroy", "nq_batch_count", "nq_instance_count",
     "nq_server_config_default", "nq_server_open", "nq_server_submit", "nq_server_poll", "nq_server_wait",
@@ -134,6 +134,8 @@ def lib() -> ctypes.CDLL:
     L.nq_config_resolve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_Config),
                                     ctypes.POINTER(_Config)]
     L.nq_generate.argtypes = [ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_uint64, _u64p]
+    L.nq_generate_device.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, _u64p,
+                                     ctypes.c_void_p]
     L.nq_count.argtypes = [ctypes.c_int, _u32p, ctypes.c_uint64, ctypes.POINTER(_Config), _u64p, _u64p, _u64p,
                            ctypes.POINTER(_Stats)]
     L.nq_count_device.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(_Config),
@@ -216,6 +218,16 @@ def generate(N: int, rows: int) -> np.ndarray:
 
 
 partial_n_queens = generate
+
+
+def generate_device(N: int, rows: int, d_tasks: int = 0, cap: int = 0, stream: int = 0) -> int:
+    """nq_generate_device: the same task list generated on the current device into the device
+    buffer at d_tasks (an int pointer to 3 * cap uint32), synchronously on `stream`.  Returns
+    the task count; with d_tasks 0 or cap too small nothing is written."""
+    n = ctypes.c_uint64()
+    _check(lib().nq_generate_device(N, rows, ctypes.c_void_p(d_tasks or None), cap, ctypes.byref(n),
+                                    ctypes.c_void_p(stream or None)))
+    return n.value
 
 
 @dataclass

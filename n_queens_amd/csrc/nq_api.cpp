@@ -538,6 +538,28 @@ int nq_generate(int N, int rows, uint32_t *tasks, uint64_t cap, uint64_t *ntasks
     return NQ_OK;
 }
 
+int nq_generate_device(int N, int rows, uint32_t *d_tasks, uint64_t cap, uint64_t *ntasks, void *stream) {
+    if (!ntasks) return fail(NQ_ERR_ARG, "ntasks is NULL");
+    if (N < 1 || N > 31) return fail(NQ_ERR_ARG, "N must be 1..31");
+    if (rows < 1 || rows > N) return fail(NQ_ERR_ARG, "rows must be 1..N");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(NQ_ERR_NO_DEVICE, "Failed to find any gpu!");
+    // Rows 0 and 1 carry the reference's restrictions: the host generates those seeds (a few
+    // hundred at most); every deeper row is unrestricted and is expanded on the device.
+    const int seed_rows = std::min(rows, 2);
+    std::vector<uint32_t> seeds;
+    generate_tasks(N, seed_rows, seeds);
+    uint64_t n = 0;
+    bool too_large = false;
+    const hipError_t e = generate_expand(N, seeds.data(), seeds.size() / 3, rows - seed_rows, d_tasks, cap, &n,
+                                         &too_large, static_cast<hipStream_t>(stream));
+    if (e == hipErrorOutOfMemory) return fail(NQ_ERR_NOMEM, "device memory for the expansion rows");
+    if (e != hipSuccess) return fail_hip(e, "generate_expand");
+    if (too_large) return fail(NQ_ERR_NOMEM, "a row above the last holds more than 2^31-1 tasks");
+    *ntasks = n;
+    return NQ_OK;
+}
+
 int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_config *cfg, uint64_t *d_result,
                     uint64_t *d_per_task, int count_nodes, void *stream) {
     if (N < 1 || N > MAX_N) return fail(NQ_ERR_ARG, "N must be 1..28 for the GPU kernel");

@@ -54,8 +54,89 @@ __global__ void batch_expand_kernel(const uint4 *__restrict__ in, uint64_t n, ui
     }
 }
 
+// A wave-uniform copy of the first active lane's 64-bit value.
+__device__ __forceinline__ uint64_t first_lane_u64(uint64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 unsigned grid_for(uint64_t n) {
     return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 8192)));
+}
+
+// A 12-byte task of the reference's list, moved as one 12-byte access.
+struct alignas(4) Task3 {
+    uint32_t c, l, r;
+};
+
+// The reference's 12-byte tasks {cur, left, right}, one row deeper, unrestricted (rows >= 2 of
+// src/n_queens.cpp:74-113 use the full mask): children per task, then every child at its
+// scanned offset, lowest column first.
+__global__ void gen_child_count_kernel(const uint32_t *__restrict__ in, uint64_t n, uint32_t last,
+                                       unsigned long long *__restrict__ counts) {
+    // n + 1 entries: counts[n] = 0 closes the scan (its offset is the row's total)
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i <= n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        if (i == n) {
+            counts[i] = 0;
+            continue;
+        }
+        const Task3 t = reinterpret_cast<const Task3 *>(in)[i];
+        counts[i] = __popc(last & ~(t.c | t.l | t.r));
+    }
+}
+
+// One wave per 64 consecutive parents, one lane per output task: the wave's children are
+// consecutive in the output, so lane k of each pass writes child k of the wave's run and the
+// 12-byte stores of a pass are contiguous (a lane writing its own parent's children would scatter
+// a store over 64 places).  A lane finds its parent by a 6-step search over the 64 offsets
+// (cross-lane reads, no LDS), then takes the rank-th candidate bit of that parent.
+__global__ void gen_expand_kernel(const uint32_t *__restrict__ in, uint64_t n, uint32_t last,
+                                  const unsigned long long *__restrict__ offsets, uint32_t *__restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+    for (uint64_t base = ((blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6) * 64; base < n;
+         base += nwaves * 64) {
+        const uint64_t i = base + lane;
+        const bool ok = i < n;
+        uint32_t c = 0, l = 0, r = 0, v = 0;
+        uint64_t o = 0;
+        if (ok) {
+            const Task3 t = reinterpret_cast<const Task3 *>(in)[i];
+            c = t.c;
+            l = t.l;
+            r = t.r;
+            v = last & ~(c | l | r);
+            o = offsets[i];
+        }
+        const uint64_t o0 = first_lane_u64(o);  // lane 0's parent exists (base < n)
+        const uint32_t lastlane = static_cast<uint32_t>(n - 1 - base < 63 ? n - 1 - base : 63);
+        uint32_t rel = ok ? static_cast<uint32_t>(o - o0) : 0u;  // at most 64 * 31 children per wave
+        const uint32_t total = __shfl(rel + static_cast<uint32_t>(__popc(v)), lastlane);
+        if (!ok) rel = total;  // never a parent below
+        for (uint32_t k = 0; k < total; k += 64) {
+            const uint32_t j = k + lane;
+            // the last parent whose run starts at or before j (its run holds j)
+            uint32_t p = 0;
+            for (uint32_t step = 32; step; step >>= 1)
+                if (static_cast<uint32_t>(__shfl(rel, p + step)) <= j) p += step;
+            uint32_t rank = j - static_cast<uint32_t>(__shfl(rel, p));
+            const uint32_t pv = __shfl(v, p), pc = __shfl(c, p), pl = __shfl(l, p), pr = __shfl(r, p);
+            // the rank-th set bit of pv (halving on popcounts)
+            uint32_t x = pv, pos = 0;
+            for (uint32_t h = 16; h; h >>= 1) {
+                const uint32_t lo = static_cast<uint32_t>(__popc(x & ((1u << h) - 1u)));
+                if (rank >= lo) {
+                    rank -= lo;
+                    x >>= h;
+                    pos += h;
+                }
+            }
+            const uint32_t q = 1u << pos;
+            if (j < total) reinterpret_cast<Task3 *>(out)[o0 + j] = Task3{pc | q, (pl | q) << 1, (pr | q) >> 1};
+        }
+    }
 }
 
 }  // namespace
@@ -113,6 +194,87 @@ hipError_t batch_expand(uint4 **tasks, uint64_t *n, uint32_t target, int levels,
         *n = total;
     }
     release();
+    return e;
+}
+
+hipError_t generate_expand(int N, const uint32_t *seeds, uint64_t nseeds, int levels, uint32_t *d_out, uint64_t cap,
+                           uint64_t *n_out, bool *too_large, hipStream_t stream) {
+    const uint32_t last = N >= 32 ? 0xFFFFFFFFu : ((1u << N) - 1u);
+    *n_out = nseeds;
+    *too_large = false;
+    if (levels <= 0 || nseeds == 0) {
+        hipError_t e = hipSuccess;
+        if (levels <= 0 && d_out && cap >= nseeds && nseeds) {
+            e = hipMemcpyAsync(d_out, seeds, nseeds * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        }
+        if (levels > 0) *n_out = 0;
+        return e;
+    }
+    // One device block per row: the row's tasks, then the scan of its child counts (m + 1
+    // entries: the last count is 0, so the last offset is the next row's size), then the scan's
+    // scratch.  A row's block is allocated once its size is known, before the row is written.
+    struct Row {
+        char *block = nullptr;
+        uint32_t *tasks = nullptr;
+        unsigned long long *counts = nullptr, *offsets = nullptr;
+        void *temp = nullptr;
+        size_t temp_bytes = 0;
+    };
+    auto round_up = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
+    auto alloc_row = [&](uint64_t m, Row *row) -> hipError_t {
+        if (m + 1 > 0x7FFFFFFFull) {  // one scan per row takes at most 2^31 - 1 items
+            *too_large = true;
+            return hipSuccess;
+        }
+        hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, row->temp_bytes, row->counts, row->offsets,
+                                                        static_cast<int>(m + 1), stream);
+        if (e != hipSuccess) return e;
+        const size_t t = round_up(m * 3 * sizeof(uint32_t)), c = round_up((m + 1) * sizeof(unsigned long long));
+        if ((e = hipMalloc(&row->block, t + 2 * c + std::max<size_t>(row->temp_bytes, 1))) != hipSuccess) return e;
+        row->tasks = reinterpret_cast<uint32_t *>(row->block);
+        row->counts = reinterpret_cast<unsigned long long *>(row->block + t);
+        row->offsets = reinterpret_cast<unsigned long long *>(row->block + t + c);
+        row->temp = row->block + t + 2 * c;
+        return hipSuccess;
+    };
+    Row cur;
+    hipError_t e = alloc_row(nseeds, &cur);
+    if (e == hipSuccess && !*too_large)
+        e = hipMemcpyAsync(cur.tasks, seeds, nseeds * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, stream);
+    uint64_t m = nseeds;
+    for (int lev = 0; e == hipSuccess && !*too_large && lev < levels; ++lev) {
+        gen_child_count_kernel<<<grid_for(m + 1), 256, 0, stream>>>(cur.tasks, m, last, cur.counts);
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        size_t bytes = cur.temp_bytes;
+        if ((e = hipcub::DeviceScan::ExclusiveSum(cur.temp, bytes, cur.counts, cur.offsets, static_cast<int>(m + 1),
+                                                  stream)) != hipSuccess)
+            break;
+        unsigned long long total = 0;
+        if ((e = hipMemcpyAsync(&total, cur.offsets + m, sizeof total, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+            break;
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) break;
+        if (lev == levels - 1) {
+            *n_out = total;
+            if (d_out && cap >= total && total) {  // else the count only
+                gen_expand_kernel<<<grid_for(m), 256, 0, stream>>>(cur.tasks, m, last, cur.offsets, d_out);
+                if ((e = hipGetLastError()) == hipSuccess) e = hipStreamSynchronize(stream);
+            }
+            break;
+        }
+        if (total == 0) {
+            *n_out = 0;
+            break;
+        }
+        Row next;
+        if ((e = alloc_row(total, &next)) != hipSuccess || *too_large) break;
+        gen_expand_kernel<<<grid_for(m), 256, 0, stream>>>(cur.tasks, m, last, cur.offsets, next.tasks);
+        if ((e = hipGetLastError()) == hipSuccess) e = hipStreamSynchronize(stream);
+        (void)hipFree(cur.block);
+        cur = next;
+        m = total;
+    }
+    if (cur.block) (void)hipFree(cur.block);
     return e;
 }
 

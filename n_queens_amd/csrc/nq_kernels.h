@@ -140,6 +140,15 @@ hipError_t serve_fit(int block, int stack_depth, int push, bool *apush, int *blo
 // and *n by its count; synchronizes the stream (nq_expand.hip).
 hipError_t batch_expand(uint4 **tasks, uint64_t *n, uint32_t target, int levels, hipStream_t stream);
 
+// The reference's task list `levels` rows below host seeds (nseeds 12-byte tasks, all at one
+// depth, rows >= 2 unrestricted), order-preserving (child count, scan, scatter per row), into
+// d_out (device, 3 * cap words) when it holds the result; *n_out = the task count either way.
+// Intermediate rows are library-owned device buffers; synchronizes the stream.  *too_large:
+// a row above the last holds more than 2^31 - 1 tasks (one scan's limit); nothing was written
+// (nq_expand.hip).
+hipError_t generate_expand(int N, const uint32_t *seeds, uint64_t nseeds, int levels, uint32_t *d_out, uint64_t cap,
+                           uint64_t *n_out, bool *too_large, hipStream_t stream);
+
 // Enqueue: zero counts, reset state, batched DFS.  Capturable into a hipGraph (kernels only).
 hipError_t launch_batch(const BatchArgs &a, hipStream_t stream);
 hipError_t batch_occupancy(int block, int stack_depth, bool apush, int *blocks_per_cu);

@@ -69,6 +69,24 @@ def test_generate_rejects_bad_args():
         nq.generate(8, 9)
 
 
+def test_generate_device_checks_args_before_the_device():
+    """nq_generate_device validates N and rows as nq_generate does, and without a device
+    reports NQ_ERR_NO_DEVICE."""
+    for N, rows in ((0, 1), (32, 2), (8, 0), (8, 9)):
+        with pytest.raises(nq.NQError) as e:
+            nq.generate_device(N, rows)
+        assert e.value.status == nq.NQ_ERR_ARG, (N, rows)
+    r = subprocess.run([sys.executable, "-c", """
+import n_queens_amd as nq
+try:
+    nq.generate_device(21, 6)
+except nq.NQError as e:
+    print("status", e.status)
+"""], capture_output=True, text=True, env=NO_GPU, cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+        timeout=120)
+    assert f"status {nq.NQ_ERR_NO_DEVICE}" in r.stdout, r.stdout + r.stderr
+
+
 def test_cpp_dropin_links_and_generates(tmp_path):
     """A reference-style caller links against libnq_hip.so through include/n_queens.h."""
     src = tmp_path / "caller.cpp"
