@@ -87,10 +87,24 @@ int main(int argc, char *argv[]) {
     gettimeofday(&start, nullptr);
     long long sum = -1;
     if (checkpoint) {
+        // One range per call, so that every counted range prints a progress line (a long run
+        // would otherwise be silent for hours); each call resumes from the file the last one
+        // wrote, after regenerating the host task list.
         nq_resume_state st;
-        if (nq_solve_resumable(N, rows, nullptr, checkpoint, range_tasks, 0, &st) == NQ_OK) {
+        int rc = NQ_OK, ranges = 0;
+        do {
+            rc = nq_solve_resumable(N, rows, nullptr, checkpoint, range_tasks, 1, &st);
+            if (rc != NQ_OK) break;
+            ranges += st.ranges;
+            if (st.ranges) {
+                print_with_time("checkpoint %s: tasks %llu of %llu done\n", checkpoint,
+                                static_cast<unsigned long long>(st.next), static_cast<unsigned long long>(st.ntasks));
+                fflush(stdout);  // seen at once also when stdout is a file or a pipe
+            }
+        } while (!st.done);
+        if (rc == NQ_OK) {
             print_with_time("checkpoint %s: %d ranges counted in this run, tasks %llu of %llu done\n", checkpoint,
-                         st.ranges, static_cast<unsigned long long>(st.next), static_cast<unsigned long long>(st.ntasks));
+                            ranges, static_cast<unsigned long long>(st.next), static_cast<unsigned long long>(st.ntasks));
             sum = static_cast<long long>(st.total);
         } else {
             fprintf(stderr, "n_queens: %s\n", nq_last_error());

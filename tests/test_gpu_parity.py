@@ -3,6 +3,7 @@
 All counts are integers and must match bit for bit (tolerance 0).
 """
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -288,12 +289,17 @@ def test_resumable_interrupt_and_resume(tmp_path, published):
 
 
 def test_cli_checkpoint_resume(tmp_path):
-    """`n_queens N rows --checkpoint PATH`: a finished checkpoint is reported by the next run."""
+    """`n_queens N rows --checkpoint PATH`: one progress line per counted range, in task order;
+    a finished checkpoint is reported by the next run."""
     p = str(tmp_path / "ck")
     r = subprocess.run([nq.CLI_PATH, "14", "4", "--checkpoint", p, "--range-tasks", "300"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0 and "cuda 14 queens result 365596, calc time:" in r.stdout, r.stdout + r.stderr
-    assert "ranges counted in this run" in r.stdout
+    n = len(nq.generate(14, 4))
+    ranges = (n + 299) // 300
+    done = [int(m) for m in re.findall(r"checkpoint \S+: tasks (\d+) of %d done" % n, r.stdout)]
+    assert done == [min(300 * (i + 1), n) for i in range(ranges)], r.stdout
+    assert f"{ranges} ranges counted in this run, tasks {n} of {n} done" in r.stdout
     r2 = subprocess.run([nq.CLI_PATH, "14", "4", "--checkpoint", p], capture_output=True, text=True, timeout=120)
     assert "0 ranges counted in this run" in r2.stdout and "cuda 14 queens result 365596" in r2.stdout
 
