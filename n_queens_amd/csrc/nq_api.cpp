@@ -343,18 +343,19 @@ static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, const 
 }
 
 
-// Rows nq_solve expands on the device (0..3): the DFS then starts dfs_rows(N) rows above the
-// last row, i.e. with dfs_rows(N) - 1 LDS frames.  Measured on one MI355X (split + DFS, final
-// loop): N=16/4 -> 2 (1.1 ms), N=17/5 -> 1 or 2 (4.0 / 4.1 ms), N=18/6 -> 1 (18.0 ms; 0: 25.0,
-// 2: 31.8), N=19/6 -> 2 (112.5 ms; 1: 118.2), N=20/6 -> 2 (881.6 ms; 1: 894.4, 3: 890.3),
-// N=21/6 -> 2 (7.36 s; 3: 7.47 s), N=22/6 -> 3 (65.5 s; 2: 69.8 s).  Up to N=20 shorter tasks
-// (the drain tail) decide, and (N + 4) / 2 rows fits.  From N=21 on, 12 frames win: the
-// unconditional push then keeps 12 waves/CU in 256-lane workgroups, while 13 frames leave the
-// masked push at 12 waves (about 7% slower) and 11 frames make 6 times more, shorter tasks.
+// Rows walked on the device below each task (0..MAX_SPLIT): the DFS then starts dfs_rows(N) rows
+// above the last row, i.e. with dfs_rows(N) - 1 LDS frames.  Measured on one MI355X (split + DFS,
+// final loop): N=16/4 -> 2 (1.1 ms), N=17/5 -> 1 or 2 (4.0 / 4.1 ms), N=18/6 -> 1 (18.0 ms; 0:
+// 25.0, 2: 31.8), N=19/6 -> 2 (112.5 ms; 1: 118.2), N=20/6 -> 2 (881.6 ms; 1: 894.4, 3: 890.3),
+// N=21/6 -> 2 (7.36 s; 3: 7.47 s), N=22/6 -> 3 (65.5 s; 2: 69.8 s), N=23/6 -> 4 (two slices of
+// 600,000 tasks: 21.8 / 35.3 s; 3: 22.4 / 36.4 s).  Up to N=20 shorter tasks (the drain tail)
+// decide, and (N + 4) / 2 rows fits.  From N=21 on, 12 frames win: the unconditional push then
+// keeps 12 waves/CU in 256-lane workgroups, while 13 frames leave the masked push at 12 waves
+// (about 3-7% slower) and 11 frames make 6 times more, shorter tasks.
 static int dfs_rows(int N) { return N <= 20 ? (N + 4) / 2 : 13; }
 
 static int auto_split(int N, int rows) {
-    return std::max(0, std::min(3, N - rows - dfs_rows(N)));
+    return std::max(0, std::min(MAX_SPLIT, N - rows - dfs_rows(N)));
 }
 
 
