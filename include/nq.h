@@ -45,7 +45,7 @@ typedef struct nq_config {
                       /* wave claims whole tasks, one at a time by default                  */
     int grid;         /* explicit workgroup count (0 = from waves_per_cu)                  */
     int split;        /* rows the DFS kernel walks below each task before its LDS DFS     */
-                      /* (0..4; -1 = automatic): a wave claims a whole task and hands the   */
+                      /* (0..8; -1 = automatic): a wave claims a whole task and hands the   */
                       /* nodes `split` rows below it to its lanes as DFS roots, in registers */
                       /* (no expanded list in memory).  Plain totals only: ignored when     */
                       /* per-task or node counts are requested.                             */
@@ -94,7 +94,7 @@ int nq_config_default(int N, int d0, nq_config *cfg);
  * else $HOME/.cache/n_queens_amd/tune.txt) holds an entry for N on the current device: its
  * preplace depth, split, block, chunk, push, waves per CU and stack depth, with a preplace depth
  * below `rows` raised to `rows` and the split shortened to keep the tuned DFS depth.  Otherwise
- * the built-in rule: preplace 0 (= rows), split = clamp(N - rows - R, 0, 4) so that the DFS
+ * the built-in rule: preplace 0 (= rows), split = clamp(N - rows - R, 0, 8) so that the DFS
  * starts R rows above the last row (R = (N+4)/2 up to N = 20, R = 13 from N = 21 on: 12 LDS
  * frames), chunk 1 with a split.  stack_depth bounds the claimed tasks (depth
  * max(rows, preplace)); the count takes the split rows off it.  No device is needed for the

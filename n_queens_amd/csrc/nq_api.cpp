@@ -204,7 +204,7 @@ static int resolve_config(int cus, int N, int d0, uint64_t ntasks, const nq_conf
 // no cfg: the automatic rule), never more than leaves the DFS two rows, at most MAX_SPLIT.
 static int effective_split(int N, int d0, const nq_config *cfg, int *split) {
     int s = (cfg && cfg->split >= 0) ? cfg->split : auto_split(N, d0);
-    if (s > MAX_SPLIT) return fail(NQ_ERR_ARG, "split must be -1 (automatic) or 0..4");
+    if (s > MAX_SPLIT) return fail(NQ_ERR_ARG, "split must be -1 (automatic) or 0..8");
     *split = std::max(0, std::min(s, N - 2 - d0));
     return NQ_OK;
 }
@@ -348,10 +348,12 @@ static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, const 
 // final loop): N=16/4 -> 2 (1.1 ms), N=17/5 -> 1 or 2 (4.0 / 4.1 ms), N=18/6 -> 1 (18.0 ms; 0:
 // 25.0, 2: 31.8), N=19/6 -> 2 (112.5 ms; 1: 118.2), N=20/6 -> 2 (881.6 ms; 1: 894.4, 3: 890.3),
 // N=21/6 -> 2 (7.36 s; 3: 7.47 s), N=22/6 -> 3 (65.5 s; 2: 69.8 s), N=23/6 -> 4 (two slices of
-// 600,000 tasks: 21.8 / 35.3 s; 3: 22.4 / 36.4 s).  Up to N=20 shorter tasks (the drain tail)
-// decide, and (N + 4) / 2 rows fits.  From N=21 on, 12 frames win: the unconditional push then
-// keeps 12 waves/CU in 256-lane workgroups, while 13 frames leave the masked push at 12 waves
-// (about 3-7% slower) and 11 frames make 6 times more, shorter tasks.
+// 600,000 tasks: 21.8 / 35.3 s; 3: 22.4 / 36.4 s), N=24/6 -> 5 (100,000 tasks: 27.1 s; 4: 27.8 s),
+// N=25/6 -> 6 (30,000 tasks: 104.0 s; 4: 113.2 s, 10 waves/CU).  Up to N=20 shorter tasks (the
+// drain tail) decide, and (N + 4) / 2 rows fits.  From N=21 on, 12 frames win: the unconditional
+// push then keeps 12 waves/CU in 256-lane workgroups, while 13 frames leave the masked push at 12
+// waves (about 3-7% slower), 14 and more fewer waves, and 11 frames make 6 times more, shorter
+// tasks.
 static int dfs_rows(int N) { return N <= 20 ? (N + 4) / 2 : 13; }
 
 static int auto_split(int N, int rows) {

@@ -424,11 +424,11 @@ __device__ __forceinline__ uint32_t wave_leader() {
 
 // Plain totals with the device split done in registers: each wave claims whole tasks from the
 // cursor (guided: a share of the remaining tasks, at least `chunk`) and walks the top S rows of a
-// task's subtree (TaskWalk, the same on
-// every lane: scalar registers); every node S rows below the task becomes one lane's DFS root
-// when that lane's previous root is exhausted.  The expanded list never exists in memory, the
-// launch needs no host synchronisation, and each task is read once.  A task's subtree stays on
-// the wave that claimed it (a few hundred roots at most, spread over its 64 lanes).  The DFS is
+// task's subtree (the same state on every lane: TaskWalk in scalar registers up to 4 rows,
+// LaneWalk with frame k in lane k beyond); every node S rows below the task becomes one lane's
+// DFS root when that lane's previous root is exhausted.  The expanded list never exists in
+// memory, the launch needs no host synchronisation, and each task is read once.  A task's
+// subtree stays on the wave that claimed it (spread over its 64 lanes).  The DFS is
 // dfs_kernel's hot loop with the per-wave board counter.
 template <bool APUSH, int S>
 __global__ void __launch_bounds__(MAX_BLOCK)
@@ -454,7 +454,7 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
     bool has = true;                      // lane still has (or may get) a DFS root
 
     // Wave-uniform state: the walk over the current task and the wave's claimed task range.
-    nq::TaskWalk<S> walk;
+    typename nq::WalkFor<S>::type walk;
     uint64_t next = 0, end = 0;
     bool drained = false;  // no task left to claim
     const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (blockDim.x >> 6);
@@ -975,7 +975,11 @@ const void *select_split_kernel(int split, bool apush) {
         {reinterpret_cast<const void *>(&split_kernel<false, 1>), reinterpret_cast<const void *>(&split_kernel<true, 1>)},
         {reinterpret_cast<const void *>(&split_kernel<false, 2>), reinterpret_cast<const void *>(&split_kernel<true, 2>)},
         {reinterpret_cast<const void *>(&split_kernel<false, 3>), reinterpret_cast<const void *>(&split_kernel<true, 3>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 4>), reinterpret_cast<const void *>(&split_kernel<true, 4>)}};
+        {reinterpret_cast<const void *>(&split_kernel<false, 4>), reinterpret_cast<const void *>(&split_kernel<true, 4>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 5>), reinterpret_cast<const void *>(&split_kernel<true, 5>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 6>), reinterpret_cast<const void *>(&split_kernel<true, 6>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 7>), reinterpret_cast<const void *>(&split_kernel<true, 7>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 8>), reinterpret_cast<const void *>(&split_kernel<true, 8>)}};
     return k[split - 1][apush ? 1 : 0];
 }
 

@@ -12,7 +12,7 @@ constexpr int MAX_N = 28;
 constexpr int MAX_BLOCK = 256;
 constexpr int MAX_WAVES_PER_BLOCK = MAX_BLOCK / 64;
 // Rows the DFS kernel can enumerate in registers below each task (split_kernel).
-constexpr int MAX_SPLIT = 4;
+constexpr int MAX_SPLIT = 8;
 
 // Device-resident per-launch state, owned by the library (one per device).
 struct DevState {

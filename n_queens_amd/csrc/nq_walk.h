@@ -2,7 +2,8 @@
 // DFS kernel does in registers (split_kernel), with no expanded task list in memory.
 //
 // A wave claims a whole task and walks the top S levels of its subtree with this state, which
-// is the same on every lane of the wave (it lives in scalar registers on the device).  Each
+// is the same on every lane of the wave (TaskWalk, S <= 4: scalar registers on the device;
+// LaneWalk, S = 5..8: one word of frame k in lane k of four vector registers).  Each
 // call to next() yields the next node exactly S rows below the task that still has a legal
 // next queen, in column order (lowest column first, the order the reference's recursion
 // visits them, src/n_queens.cpp:15-20); the wave hands it to one of its lanes as that lane's
@@ -35,6 +36,7 @@ NQ_HD uint32_t walk_popc(uint32_t x) {
 template <int S>
 struct TaskWalk {
     static_assert(S >= 1 && S <= 4, "1..4 rows are enumerated in registers");
+    static constexpr int rows = S;
     // Frame k is the node k rows below the task (frame 0 is the task): its columns, diagonals and
     // the candidates not yet taken.  Frames 0..top are valid.
     uint32_t c[S], l[S], r[S], v[S];
@@ -98,6 +100,92 @@ struct TaskWalk {
             ++top;
         }
     }
+};
+
+// One 32-bit word of every frame of a deep walk.  On the device frame k's word sits in lane k of
+// one register of the wave (x[0]): read with readlane at a wave-uniform index, written by a
+// select on the lane id (a compare and a v_cndmask).  Each access costs the same at any depth,
+// where TaskWalk's selects over S scalar registers grow with S (and S = 4 already fills the
+// scalar register file).  On the host (the CPU model) frame k is x[k].
+struct LaneWord {
+    uint32_t x[8];
+    NQ_HD uint32_t get(int k) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x[0]), k));
+#else
+        return x[k];
+#endif
+    }
+    NQ_HD void set(int k, uint32_t val) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+        x[0] = lane == static_cast<uint32_t>(k) ? val : x[0];
+#else
+        x[k] = val;
+#endif
+    }
+};
+
+// TaskWalk with the frames in lanes (LaneWord), used for 5..8 rows: the same nodes in the same
+// order, the same *direct counts.  Called by every lane of the wave with the same arguments (the
+// state is uniform; only frame k's word lives in lane k).
+template <int S>
+struct LaneWalk {
+    static_assert(S >= 1 && S <= 8, "1..8 rows are enumerated with the frames in lanes");
+    static constexpr int rows = S;
+    LaneWord c, l, r, v;
+    int top = -1;
+    uint32_t d0 = 0;
+
+    NQ_HD void start(uint32_t c0, uint32_t l0, uint32_t r0, uint32_t v0, uint32_t d) {
+        c.x[0] = l.x[0] = r.x[0] = v.x[0] = 0;
+        c.set(0, c0);
+        l.set(0, l0);
+        r.set(0, r0);
+        v.set(0, v0);
+        top = 0;
+        d0 = d;
+    }
+
+    NQ_HD bool next(uint32_t N, uint32_t last, uint32_t *oc, uint32_t *ol, uint32_t *orr, uint32_t *ov,
+                    uint64_t *direct) {
+        for (;;) {
+            uint32_t fv = 0;
+            while (top >= 0 && (fv = v.get(top)) == 0) --top;  // drop exhausted frames
+            if (top < 0) return false;
+            const uint32_t p = fv & (0u - fv);
+            v.set(top, fv ^ p);
+            const uint32_t cc = c.get(top) | p, cl = (l.get(top) | p) << 1, cr = (r.get(top) | p) >> 1;
+            const uint32_t cv = last & ~(cc | cl | cr);
+            if (cv == 0) continue;
+            if (d0 + static_cast<uint32_t>(top) + 2u == N) {  // the child is on the last row
+                *direct += walk_popc(cv);
+                continue;
+            }
+            if (top + 1 == S) {
+                *oc = cc;
+                *ol = cl;
+                *orr = cr;
+                *ov = cv;
+                return true;
+            }
+            ++top;
+            c.set(top, cc);
+            l.set(top, cl);
+            r.set(top, cr);
+            v.set(top, cv);
+        }
+    }
+};
+
+// The walk split_kernel uses for S rows.
+template <int S, bool DEEP = (S > 4)>
+struct WalkFor {
+    using type = TaskWalk<S>;
+};
+template <int S>
+struct WalkFor<S, true> {
+    using type = LaneWalk<S>;
 };
 
 }  // namespace nq

@@ -12,10 +12,10 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("N,rows", [(12, 2), (14, 3), (16, 4), (17, 5), (18, 6)])
 def test_split_kernel_totals(N, rows, published):
-    """Every split depth 1..4, both push forms, narrow and wide workgroups, single and chunked
+    """Every split depth 1..8 (scalar-register walk up to 4, frames in lanes beyond), both push forms, narrow and wide workgroups, single and chunked
     task claims: the same total as A000170."""
     t = nq.generate(N, rows)
-    for split in (1, 2, 3, 4):
+    for split in range(1, 9):
         for push, block, chunk in ((nq.NQ_PUSH_MASKED, 64, 1), (nq.NQ_PUSH_ALWAYS, 256, 5), (0, 0, 0)):
             r = nq.count(N, t, cfg=nq.Config(split=split, push=push, block=block, chunk=chunk))
             assert 2 * r.sum == published["solutions"][N], (N, rows, split, push, block, chunk)
@@ -31,7 +31,7 @@ def test_split_kernel_mixed_depths_vs_oracle():
     t = np.concatenate([t, np.zeros((1, 3), np.uint32), np.array([[0b11, 0b1100, 0]], np.uint32)])
     rng.shuffle(t)
     want = oracle.count_tasks(N, t)
-    for split in range(0, 5):
+    for split in range(0, 9):
         assert nq.count(N, t, cfg=nq.Config(split=split)).sum == want, split
 
 
@@ -46,7 +46,7 @@ def test_split_ignored_for_per_task_and_node_counts():
 
 def test_split_above_the_limit_is_an_argument_error():
     with pytest.raises(nq.NQError) as e:
-        nq.count(12, nq.generate(12, 2), cfg=nq.Config(split=5))
+        nq.count(12, nq.generate(12, 2), cfg=nq.Config(split=9))
     assert e.value.status == nq.NQ_ERR_ARG
 
 

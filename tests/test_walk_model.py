@@ -1,5 +1,5 @@
-"""CPU model of the device split in registers (split_kernel): the kernel's own TaskWalk
-(n_queens_amd/csrc/nq_walk.h) inside an emulation of its claim/refill control flow
+"""CPU model of the device split in registers (split_kernel): the kernel's own TaskWalk and
+LaneWalk (n_queens_amd/csrc/nq_walk.h) inside an emulation of its claim/refill control flow
 (tests/walk_model.cpp).  For every case the model must claim each task exactly once, hand out
 only roots that fit the configured stack, terminate, and reach the oracle's total."""
 import itertools
@@ -22,11 +22,11 @@ def model(tmp_path_factory):
     return str(exe)
 
 
-def run_model(model, tmp_path, N, tasks, split, chunk, waves, stack):
+def run_model(model, tmp_path, N, tasks, split, chunk, waves, stack, lanes=False):
     f = tmp_path / "tasks.bin"
     np.ascontiguousarray(tasks, dtype=np.uint32).tofile(f)
-    out = subprocess.check_output([model, str(N), str(split), str(chunk), str(waves), str(stack), str(f)],
-                                  timeout=120).split()
+    out = subprocess.check_output([model, str(N), str(split), str(chunk), str(waves), str(stack), str(f)]
+                                  + (["lanes"] if lanes else []), timeout=120).split()
     total, flags, roots, claims_ok, frames_ok, done, cursor_claims = (int(x) for x in out)
     run_model.cursor_claims = cursor_claims
     return total, flags, roots, claims_ok == 1, frames_ok == 1, done == 1
@@ -34,7 +34,7 @@ def run_model(model, tmp_path, N, tasks, split, chunk, waves, stack):
 
 @pytest.mark.parametrize("N", range(5, 14))
 def test_walk_model_totals(model, tmp_path, N):
-    for rows, split in itertools.product(range(1, N - 1), range(1, 5)):
+    for rows, split in itertools.product(range(1, N - 1), range(1, 9)):
         tasks = oracle.gen_tasks(N, rows)
         want = oracle.count_tasks(N, tasks)
         stack = max(1, N - 1 - (rows + split))
@@ -55,10 +55,22 @@ def test_walk_model_mixed_depths_and_edge_tasks(model, tmp_path):
     tasks = np.concatenate([tasks, np.array([[0b11, 0b1100, 0], [0, 0, 0]], dtype=np.uint32)])  # dead, empty
     rng.shuffle(tasks)
     want = oracle.count_tasks(N, tasks)
-    for split in range(1, 5):
+    for split, lanes in itertools.chain(((s, False) for s in range(1, 9)), ((s, True) for s in range(1, 5))):
         stack = N - 1 - split  # the empty board (depth 0) is the shallowest
-        total, flags, _, claims, frames, done = run_model(model, tmp_path, N, tasks, split, 2, 5, stack)
-        assert (claims, frames, done, flags, total) == (True, True, True, 0, want), split
+        total, flags, _, claims, frames, done = run_model(model, tmp_path, N, tasks, split, 2, 5, stack, lanes)
+        assert (claims, frames, done, flags, total) == (True, True, True, 0, want), (split, lanes)
+
+
+@pytest.mark.parametrize("N", [9, 12])
+def test_lane_walk_equals_register_walk(model, tmp_path, N):
+    """The walk with its frames in lanes (split_kernel's walk beyond 4 rows) run at 1..4 rows,
+    where the scalar-register walk is the reference: the same totals, roots and claims."""
+    for rows, split in itertools.product(range(1, N - 2), range(1, 5)):
+        tasks = oracle.gen_tasks(N, rows)
+        stack = max(1, N - 1 - (rows + split))
+        a = run_model(model, tmp_path, N, tasks, split, 3, 7, stack)
+        b = run_model(model, tmp_path, N, tasks, split, 3, 7, stack, lanes=True)
+        assert a == b and a[0] == oracle.count_tasks(N, tasks), (N, rows, split)
 
 
 def test_walk_model_guided_claims_are_few(model, tmp_path):

@@ -8,6 +8,9 @@
 // a plain recursion).  Waves advance round-robin, one refill + trips-until-a-lane-parks at a
 // time, so lanes finish at different times and waves interleave their claims.
 //
+// The walk is TaskWalk (scalar frames, split <= 4) or LaneWalk (frames in lanes, split 5..8), as
+// split_kernel picks them; a seventh argument `lanes` runs LaneWalk at every split.
+//
 // Checks, reported on stdout as one line "total flags roots claims_ok frames_ok terminated
 // cursor_claims":
 //   * every task index is claimed exactly once;
@@ -15,10 +18,11 @@
 //   * every wave terminates.
 // The total is compared with the oracle by tests/test_walk_model.py.
 //
-//   walk_model N split chunk waves stack_depth taskfile     (taskfile: uint32 triples)
+//   walk_model N split chunk waves stack_depth taskfile [lanes]     (taskfile: uint32 triples)
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <vector>
 
@@ -61,10 +65,11 @@ struct Model {
     bool frames_ok = true;
 };
 
-template <int S>
+template <typename W>
 struct Wave {
+    static constexpr int S = W::rows;
     Lane lane[64];
-    nq::TaskWalk<S> walk;
+    W walk;
     uint64_t next = 0, end = 0;
     bool drained = false, finished = false;
     uint64_t wsum = 0;
@@ -153,9 +158,9 @@ struct Wave {
     }
 };
 
-template <int S>
+template <typename W>
 int run(Model &m, int nwaves) {
-    std::vector<Wave<S>> waves(nwaves);
+    std::vector<Wave<W>> waves(nwaves);
     m.nwaves = static_cast<uint64_t>(nwaves);
     // Waves take turns; a wave's turn count is bounded by its claims plus its roots, so a
     // generous cap detects a wave that never terminates.
@@ -182,10 +187,11 @@ int run(Model &m, int nwaves) {
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc != 7) {
-        fprintf(stderr, "usage: %s N split chunk waves stack_depth taskfile\n", argv[0]);
+    if (argc != 7 && argc != 8) {
+        fprintf(stderr, "usage: %s N split chunk waves stack_depth taskfile [lanes]\n", argv[0]);
         return 2;
     }
+    const bool lanes = argc == 8 && !strcmp(argv[7], "lanes");
     Model m;
     m.N = static_cast<uint32_t>(atoi(argv[1]));
     const int split = atoi(argv[2]);
@@ -202,11 +208,21 @@ int main(int argc, char **argv) {
     m.tasks = &tasks;
     m.ntasks = tasks.size() / 3;
     m.claimed.assign(m.ntasks, 0);
+    if (lanes) switch (split) {
+            case 1: return run<nq::LaneWalk<1>>(m, nwaves);
+            case 2: return run<nq::LaneWalk<2>>(m, nwaves);
+            case 3: return run<nq::LaneWalk<3>>(m, nwaves);
+            case 4: return run<nq::LaneWalk<4>>(m, nwaves);
+        }
     switch (split) {
-        case 1: return run<1>(m, nwaves);
-        case 2: return run<2>(m, nwaves);
-        case 3: return run<3>(m, nwaves);
-        case 4: return run<4>(m, nwaves);
+        case 1: return run<nq::WalkFor<1>::type>(m, nwaves);
+        case 2: return run<nq::WalkFor<2>::type>(m, nwaves);
+        case 3: return run<nq::WalkFor<3>::type>(m, nwaves);
+        case 4: return run<nq::WalkFor<4>::type>(m, nwaves);
+        case 5: return run<nq::WalkFor<5>::type>(m, nwaves);
+        case 6: return run<nq::WalkFor<6>::type>(m, nwaves);
+        case 7: return run<nq::WalkFor<7>::type>(m, nwaves);
+        case 8: return run<nq::WalkFor<8>::type>(m, nwaves);
     }
     return 2;
 }
