@@ -34,10 +34,10 @@ METRIC = "N=21 wall s + DFS nodes/s on 1 GPU; batched-mix instances/s vs per-lau
 OPS_PER_NODE = 11          # p=v&-v (2), v^=p (1), c|=p (1), l=(l|p)<<1 (2), r=(r|p)>>1 (2), v'=last&~(c|l|r) (3)
 PEAK_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
 LANES_PER_CU = 64          # int32 VALU lanes per CU per clock (4 SIMD x 16)
-# HBM bytes per N=21 rows-6 launch of the DFS kernel (split_kernel, 2 rows walked in registers),
+# HBM bytes per N=21 rows-6 launch of the DFS kernel (split_kernel, 3 rows walked in registers),
 # from two rocprofv3 --pmc passes scaled by a 256 MiB copy (tests/probes/copy_probe: FETCH_SIZE
 # x2.000, WRITE_SIZE x1.000): profiles/split_kernel_n21_rows6_traffic.md.
-TRAFFIC_N21_ROWS6 = round((2 * 62471.6 + 2975.0) * 1024)
+TRAFFIC_N21_ROWS6 = round((2 * 63816.6875 + 3429.6875) * 1024)
 
 
 def log(msg: str) -> None:
@@ -419,7 +419,7 @@ def main() -> int:
                      # What binds the loop, from the SQ counters of this launch
                      # (profiles/split_kernel_n21_rows6_sq_counters.md), and the CU cycles one
                      # wave spends per queen placed (nodes and completed boards).
-                     "binding": "VALU issue, 0.95 of the wave-instruction peak: 20 VALU, one ds_write_b128 "
+                     "binding": "VALU issue, 0.97 of the wave-instruction peak: 20 VALU, one ds_write_b128 "
                                 "push and one ds_read_b128 pop per iteration, 1.37 placements per iteration "
                                 "(forced moves)",
                      "cu_cycles_per_wave_placement": (ms_per_step / 1e3) * PEAK_CLOCK_HZ * info["cus"]

@@ -350,11 +350,13 @@ static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, const 
 // N=21/6 -> 2 (7.36 s; 3: 7.47 s), N=22/6 -> 3 (65.5 s; 2: 69.8 s), N=23/6 -> 4 (two slices of
 // 600,000 tasks: 21.8 / 35.3 s; 3: 22.4 / 36.4 s), N=24/6 -> 5 (100,000 tasks: 27.1 s; 4: 27.8 s),
 // N=25/6 -> 6 (30,000 tasks: 104.0 s; 4: 113.2 s, 10 waves/CU).  Up to N=20 shorter tasks (the
-// drain tail) decide, and (N + 4) / 2 rows fits.  From N=21 on, 12 frames win: the unconditional
+// drain tail) decide, and (N + 4) / 2 rows fits.  From N=22 on, 12 frames win: the unconditional
 // push then keeps 12 waves/CU in 256-lane workgroups, while 13 frames leave the masked push at 12
-// waves (about 3-7% slower), 14 and more fewer waves, and 11 frames make 6 times more, shorter
-// tasks.
-static int dfs_rows(int N) { return N <= 20 ? (N + 4) / 2 : 13; }
+// waves (about 3-7% slower), 14 and more fewer waves, and 11 frames (the masked push at 14 waves/CU
+// in 128-lane workgroups, on roots with a sixth of the work) lose 0.5 % at N=22 (1M tasks, 4 rows
+// walked: 9.21 s against 9.16), 2.8 % at N=23 and 6.6 % at N=24.  At N=21 the 11 frames win by
+// 0.6 % (3 rows walked: 6,297 ms against 6,335 with 2).
+static int dfs_rows(int N) { return N <= 20 ? (N + 4) / 2 : N == 21 ? 12 : 13; }
 
 static int auto_split(int N, int rows) {
     return std::max(0, std::min(MAX_SPLIT, N - rows - dfs_rows(N)));

@@ -424,7 +424,7 @@ __device__ __forceinline__ uint32_t wave_leader() {
 
 // Plain totals with the device split done in registers: each wave claims whole tasks from the
 // cursor (guided: a share of the remaining tasks, at least `chunk`) and walks the top S rows of a
-// task's subtree (the same state on every lane: TaskWalk in scalar registers up to 4 rows,
+// task's subtree (the same state on every lane: TaskWalk in scalar registers for 1-2 rows,
 // LaneWalk with frame k in lane k beyond); every node S rows below the task becomes one lane's
 // DFS root when that lane's previous root is exhausted.  The expanded list never exists in
 // memory, the launch needs no host synchronisation, and each task is read once.  A task's

@@ -2,8 +2,8 @@
 // DFS kernel does in registers (split_kernel), with no expanded task list in memory.
 //
 // A wave claims a whole task and walks the top S levels of its subtree with this state, which
-// is the same on every lane of the wave (TaskWalk, S <= 4: scalar registers on the device;
-// LaneWalk, S = 5..8: one word of frame k in lane k of four vector registers).  Each
+// is the same on every lane of the wave (TaskWalk: scalar registers on the device; LaneWalk: one
+// word of frame k in lane k of four vector registers; WalkFor picks one by depth).  Each
 // call to next() yields the next node exactly S rows below the task that still has a legal
 // next queen, in column order (lowest column first, the order the reference's recursion
 // visits them, src/n_queens.cpp:15-20); the wave hands it to one of its lanes as that lane's
@@ -126,7 +126,7 @@ struct LaneWord {
     }
 };
 
-// TaskWalk with the frames in lanes (LaneWord), used for 5..8 rows: the same nodes in the same
+// TaskWalk with the frames in lanes (LaneWord), used for 3..8 rows: the same nodes in the same
 // order, the same *direct counts.  Called by every lane of the wave with the same arguments (the
 // state is uniform; only frame k's word lives in lane k).
 template <int S>
@@ -178,8 +178,10 @@ struct LaneWalk {
     }
 };
 
-// The walk split_kernel uses for S rows.
-template <int S, bool DEEP = (S > 4)>
+// The walk split_kernel uses for S rows: scalar registers for 1 and 2 rows, frames in lanes from 3
+// on (N=21 rows 6: 3 rows 6,297 ms against 6,338 with TaskWalk<3>; N=23 rows 6, 300,000 tasks: 4
+// rows 15.42 s against 15.47; 2 rows alike either way).
+template <int S, bool DEEP = (S > 2)>
 struct WalkFor {
     using type = TaskWalk<S>;
 };

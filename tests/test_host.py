@@ -306,11 +306,11 @@ except nq.NQError as e:
 def test_default_and_solve_configs_leave_block_automatic():
     """nq_config_default / nq_solve_config need no device: workgroup size 0 (automatic, chosen by
     occupancy at launch), the smallest legal stack, chunk 16, automatic push, and the split rule
-    clamp(N - rows - R, 0, 8) with R = (N+4)/2 up to N=20 and 13 from N=21 on."""
+    clamp(N - rows - R, 0, 8) with R = (N+4)/2 up to N=20, 12 at N=21 and 13 from N=22 on."""
     c = nq.config_default(21, 9)
     assert (c.block, c.stack_depth, c.chunk, c.push, c.split) == (0, 11, 16, nq.NQ_PUSH_AUTO, 0)
-    for N, rows, split in ((21, 6, 2), (22, 6, 3), (20, 6, 2), (19, 6, 2), (18, 6, 1), (17, 5, 2), (16, 4, 2),
-                           (13, 5, 0), (23, 6, 4), (21, 4, 4), (24, 6, 5), (27, 7, 7), (28, 7, 8), (28, 2, 8), (24, 10, 1)):
+    for N, rows, split in ((21, 6, 3), (22, 6, 3), (20, 6, 2), (19, 6, 2), (18, 6, 1), (17, 5, 2), (16, 4, 2),
+                           (13, 5, 0), (23, 6, 4), (21, 4, 5), (24, 6, 5), (27, 7, 7), (28, 7, 8), (28, 2, 8), (24, 10, 1)):
         s = nq.solve_config(N, rows)
         assert s.split == split and s.block == 0 and s.stack_depth == N - 1 - rows, (N, rows)
 

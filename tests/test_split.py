@@ -95,13 +95,13 @@ def test_count_device_split_is_captured_in_a_graph(published):
 def test_reference_single_gpu_setting_and_automatic_splits(published, monkeypatch):
     """The reference's advice for one GPU (README: N <= 22 with 5 rows pre-placed under
     configuration3) through the CLI at N=20, and N=21 with 5 and 4 rows, where the automatic split
-    walks 3 and 4 rows so that the DFS keeps 12 LDS frames (tuner cache off: built-in rule)."""
+    walks 4 and 5 rows so that the DFS keeps 11 LDS frames (tuner cache off: built-in rule)."""
     import subprocess
     monkeypatch.setenv("NQ_TUNE_CACHE", "")
     p = subprocess.run([nq.CLI_PATH, "20", "5", "--config", "3"], capture_output=True, text=True, timeout=120)
     assert p.returncode == 0 and "cuda 20 queens result 39029188884, calc time:" in p.stdout, p.stdout + p.stderr
-    for rows, split in ((5, 3), (4, 4)):
+    for rows, split in ((5, 4), (4, 5)):
         cfg = nq.solve_config(21, rows)
-        assert cfg.split == split and 21 - 1 - (rows + split) == 12, (rows, cfg)
+        assert cfg.split == split and 21 - 1 - (rows + split) == 11, (rows, cfg)
         total, _ = nq.solve(21, rows)
         assert total == published["solutions"][21], rows
