@@ -230,8 +230,8 @@ typedef struct nq_server_config {
                                  /* the LDS stack and one task's run time on its lane      */
     int block;                   /* workgroup size (default 128)                           */
     int chunk;                   /* tickets a wave claims at once while published work is  */
-                                 /* plentiful, 1..64 (default 64); else one per idle lane  */
-    double batch_work;           /* estimated DFS nodes per published slice (default 1e9)  */
+                                 /* plentiful, 1..64 (default 32); else one per idle lane  */
+    double batch_work;           /* estimated DFS nodes per published slice (default 5e8)  */
 } nq_server_config;
 
 typedef struct nq_request_stats {
