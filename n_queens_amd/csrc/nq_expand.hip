@@ -87,14 +87,19 @@ __global__ void gen_child_count_kernel(const uint32_t *__restrict__ in, uint64_t
     }
 }
 
-// One wave per 64 consecutive parents, one lane per output task: the wave's children are
-// consecutive in the output, so lane k of each pass writes child k of the wave's run and the
-// 12-byte stores of a pass are contiguous (a lane writing its own parent's children would scatter
-// a store over 64 places).  A lane finds its parent by a 6-step search over the 64 offsets
-// (cross-lane reads, no LDS), then takes the rank-th candidate bit of that parent.
-__global__ void gen_expand_kernel(const uint32_t *__restrict__ in, uint64_t n, uint32_t last,
+// One wave per 64 consecutive parents.  Every lane writes its parent's children, lowest column
+// first, into the wave's staging buffer in LDS at their place in the wave's run (the scan's
+// offsets relative to the wave's first), then the wave copies the buffer out with lane k of each
+// pass storing child k: contiguous 12-byte stores, as if one lane made each child, without the
+// cross-lane search for the parent.  A run longer than the buffer goes through it in windows.
+// Launched with 256 lanes: four waves, one stage each.
+constexpr uint32_t GEN_STAGE = 1024;  // children staged per wave: 12 KiB of LDS, 12 waves per CU
+
+__global__ void __launch_bounds__(256) gen_expand_kernel(const uint32_t *__restrict__ in, uint64_t n, uint32_t last,
                                   const unsigned long long *__restrict__ offsets, uint32_t *__restrict__ out) {
+    __shared__ Task3 stage[4][GEN_STAGE];
     const uint32_t lane = threadIdx.x & 63u;
+    Task3 *buf = stage[threadIdx.x >> 6];
     const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
     for (uint64_t base = ((blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6) * 64; base < n;
          base += nwaves * 64) {
@@ -112,29 +117,26 @@ __global__ void gen_expand_kernel(const uint32_t *__restrict__ in, uint64_t n, u
         }
         const uint64_t o0 = first_lane_u64(o);  // lane 0's parent exists (base < n)
         const uint32_t lastlane = static_cast<uint32_t>(n - 1 - base < 63 ? n - 1 - base : 63);
-        uint32_t rel = ok ? static_cast<uint32_t>(o - o0) : 0u;  // at most 64 * 31 children per wave
+        const uint32_t rel = ok ? static_cast<uint32_t>(o - o0) : 0u;  // at most 64 * 31 children per wave
         const uint32_t total = __shfl(rel + static_cast<uint32_t>(__popc(v)), lastlane);
-        if (!ok) rel = total;  // never a parent below
-        for (uint32_t k = 0; k < total; k += 64) {
-            const uint32_t j = k + lane;
-            // the last parent whose run starts at or before j (its run holds j)
-            uint32_t p = 0;
-            for (uint32_t step = 32; step; step >>= 1)
-                if (static_cast<uint32_t>(__shfl(rel, p + step)) <= j) p += step;
-            uint32_t rank = j - static_cast<uint32_t>(__shfl(rel, p));
-            const uint32_t pv = __shfl(v, p), pc = __shfl(c, p), pl = __shfl(l, p), pr = __shfl(r, p);
-            // the rank-th set bit of pv (halving on popcounts)
-            uint32_t x = pv, pos = 0;
-            for (uint32_t h = 16; h; h >>= 1) {
-                const uint32_t lo = static_cast<uint32_t>(__popc(x & ((1u << h) - 1u)));
-                if (rank >= lo) {
-                    rank -= lo;
-                    x >>= h;
-                    pos += h;
-                }
+        for (uint32_t w = 0; w < total; w += GEN_STAGE) {
+            // this lane's children with places in [w, w + GEN_STAGE)
+            uint32_t x = v, pos = rel;
+            while (x && pos < w + GEN_STAGE) {
+                const uint32_t q = x & (0u - x);
+                x ^= q;
+                if (pos >= w) buf[pos - w] = Task3{c | q, (l | q) << 1, (r | q) >> 1};
+                ++pos;
             }
-            const uint32_t q = 1u << pos;
-            if (j < total) reinterpret_cast<Task3 *>(out)[o0 + j] = Task3{pc | q, (pl | q) << 1, (pr | q) >> 1};
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t len = total - w < GEN_STAGE ? total - w : GEN_STAGE;
+            Task3 *dst = reinterpret_cast<Task3 *>(out) + o0 + w;
+            for (uint32_t j = lane; j < len; j += 64) dst[j] = buf[j];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
 }
