@@ -347,15 +347,15 @@ static int enqueue_split(Device *d, int N, int d0, const uint32_t *tasks, const 
 // above the last row, i.e. with dfs_rows(N) - 1 LDS frames.  Measured on one MI355X (split + DFS,
 // final loop): N=16/4 -> 2 (1.1 ms), N=17/5 -> 1 or 2 (4.0 / 4.1 ms), N=18/6 -> 1 (18.0 ms; 0:
 // 25.0, 2: 31.8), N=19/6 -> 2 (112.5 ms; 1: 118.2), N=20/6 -> 2 (881.6 ms; 1: 894.4, 3: 890.3),
-// N=21/6 -> 2 (7.36 s; 3: 7.47 s), N=22/6 -> 3 (65.5 s; 2: 69.8 s), N=23/6 -> 4 (two slices of
+// N=21/6 -> 3 (6,297 ms; 2: 6,335 ms; before the walk kept its frames in lanes 2 won, 7.36 s
+// against 7.47 s), N=22/6 -> 3 (65.5 s; 2: 69.8 s), N=23/6 -> 4 (two slices of
 // 600,000 tasks: 21.8 / 35.3 s; 3: 22.4 / 36.4 s), N=24/6 -> 5 (100,000 tasks: 27.1 s; 4: 27.8 s),
 // N=25/6 -> 6 (30,000 tasks: 104.0 s; 4: 113.2 s, 10 waves/CU).  Up to N=20 shorter tasks (the
 // drain tail) decide, and (N + 4) / 2 rows fits.  From N=22 on, 12 frames win: the unconditional
 // push then keeps 12 waves/CU in 256-lane workgroups, while 13 frames leave the masked push at 12
 // waves (about 3-7% slower), 14 and more fewer waves, and 11 frames (the masked push at 14 waves/CU
 // in 128-lane workgroups, on roots with a sixth of the work) lose 0.5 % at N=22 (1M tasks, 4 rows
-// walked: 9.21 s against 9.16), 2.8 % at N=23 and 6.6 % at N=24.  At N=21 the 11 frames win by
-// 0.6 % (3 rows walked: 6,297 ms against 6,335 with 2).
+// walked: 9.21 s against 9.16), 2.8 % at N=23 and 6.6 % at N=24.  At N=21 the 11 frames win.
 static int dfs_rows(int N) { return N <= 20 ? (N + 4) / 2 : N == 21 ? 12 : 13; }
 
 static int auto_split(int N, int rows) {
