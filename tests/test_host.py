@@ -427,3 +427,19 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert p.returncode == 2
     assert "one GPU" in p.stderr and p.stdout == ""
+
+
+def test_cli_checkpoint_refusals_without_gpu(tmp_path):
+    """`--checkpoint` counts one range per call of nq_solve_resumable: a checkpoint of another
+    problem is refused before any device work, and with no device the call fails after writing
+    the initial checkpoint; either way the CLI prints the library's message and result -1, and no
+    progress line."""
+    foreign = tmp_path / "foreign"
+    foreign.write_text("nq-checkpoint 1\nN 14 rows 4 ntasks 4816 next 0 sum 0\n")
+    p = run_cli(13, 5, "--checkpoint", str(foreign))
+    assert "checkpoint is for a different problem" in p.stderr, p.stderr
+    assert re.search(r"cuda 13 queens result -1, calc time:", p.stdout) and "tasks" not in p.stdout
+    fresh = tmp_path / "fresh"
+    p = run_cli(13, 5, "--checkpoint", str(fresh), "--range-tasks", "1000")
+    assert re.search(r"cuda 13 queens result -1, calc time:", p.stdout) and "done" not in p.stdout
+    assert fresh.read_text() == "nq-checkpoint 1\nN 13 rows 5 ntasks 15550 next 0 sum 0\n"
