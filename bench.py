@@ -13,6 +13,8 @@ placed at depths rows+1..N-1).  Also reported:
   * cpu_baseline: the reference's own OpenMP path (oracle/_ref, parallel_n_queens, which fixes
     32 threads) timed on this host on a bounded sample, and cpu_baseline_port: the oracle's
     restatement on every usable CPU of this host (one thread per CPU), same sample;
+  * per_task_counts: the same count with one count per task (the reference kernel's own output,
+    partial_sum), timed once, its sum checked against the total;
   * batched_mix and serving: the 512-instance mix and the 400 Poisson requests, every count
     checked against the oracle's (tests/golden/instance_counts.json).
 Prints ONE JSON line on stdout.
@@ -391,6 +393,29 @@ def main() -> int:
         return 1
     ms_per_step = sum(step_ms) / len(step_ms)
     nodes_per_s = nodes / (ms_per_step / 1e3)
+
+    per_task = None
+    if not args.no_nodes:
+        # The reference kernel's own output, one count per task (partial_sum,
+        # src/n_queens_cuda.cu:79), through the same split: one run, HIP events on the stream.
+        d_per_task = torch.zeros(ntasks, dtype=torch.int64, device=dev)
+        d_result.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        nq.count_device(N, d_tasks.data_ptr(), ntasks, d_result.data_ptr(), d_per_task=d_per_task.data_ptr(), cfg=cfg,
+                        stream=stream.cuda_stream)
+        e1.record(stream)
+        stream.synchronize()
+        pt = d_per_task.cpu().numpy().view(np.uint64)
+        pt_ok = int(pt.sum()) == sum0 == int(d_result.cpu().numpy().view(np.uint64)[0])
+        per_task = {"ms": e0.elapsed_time(e1), "tasks": ntasks, "sum_matches_total": pt_ok,
+                    "kernel": "split_kernel with per-task accumulation (the reference kernel's partial_sum)"}
+        log(f"per-task counts: {per_task['ms']:.1f} ms, sum matches the total: {pt_ok}")
+        del d_per_task
+        if not pt_ok:
+            log("ERROR: per-task counts do not sum to the total")
+            return 1
     peak = info["cus"] * LANES_PER_CU * PEAK_CLOCK_HZ / 1e12
     achieved = nodes * OPS_PER_NODE / (ms_per_step / 1e3) / 1e12
     # The DFS launch of a timed step as the library resolves it (the split tasks start at depth
@@ -410,7 +435,7 @@ def main() -> int:
                    "waves_per_cu": launch.grid * (launch.block // 64) // info["cus"],
                    "chunk": launch.chunk, "device_split_rows": cfg.split,
                    "parallelism": "single GPU, persistent grid"},
-        "solutions": 2 * sum0, "nodes": nodes,
+        "solutions": 2 * sum0, "nodes": nodes, "per_task_counts": per_task,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tops/s (int32 lane-ops)",
                      "frac": achieved / peak,
                      "traffic": TRAFFIC_N21_ROWS6 if (N, rows) == (21, 6) else None,

@@ -430,10 +430,18 @@ __device__ __forceinline__ uint32_t wave_leader() {
 // memory, the launch needs no host synchronisation, and each task is read once.  A task's
 // subtree stays on the wave that claimed it (spread over its 64 lanes).  The DFS is
 // dfs_kernel's hot loop with the per-wave board counter.
-template <bool APUSH, int S>
+//
+// PT: per-task counts as well (the reference kernel's partial_sum, src/n_queens_cuda.cu:79), into
+// per_task[], zeroed before the launch.  The hot loop then counts per lane; a lane keeps the task
+// of its current root and the completions of that task's roots it has run, and adds them to
+// per_task[] with one atomic when it takes a root of another task (a wave walks its tasks one after
+// another, so a lane's roots come in task order) and at exit.  The walk's last-row counts go to
+// their task when the wave moves on to the next task, edge tasks' counts when they are claimed.
+template <bool APUSH, int S, bool PT>
 __global__ void __launch_bounds__(MAX_BLOCK)
 split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint32_t chunk, uint32_t stack_depth,
-             DevState *__restrict__ st, uint64_t *__restrict__ trace, const unsigned long long *__restrict__ ntasks_dev) {
+             DevState *__restrict__ st, uint64_t *__restrict__ per_task, uint64_t *__restrict__ trace,
+             const unsigned long long *__restrict__ ntasks_dev) {
     if (ntasks_dev) ntasks = min(ntasks, static_cast<uint64_t>(*ntasks_dev));
     extern __shared__ uint4 lds[];  // lds_layout(): dead frame, stack, wq (unused), red
     if (!lds_base_ok(lds, st)) return;
@@ -452,6 +460,11 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
     uint32_t wcnt = 0;                    // completed boards of the wave since the last fold
     uint64_t wsum = 0;                    // the wave's count: folded wcnt + last-row and edge counts
     bool has = true;                      // lane still has (or may get) a DFS root
+    // PT, per lane: the task of the current root, the completions of that task's roots not yet
+    // added to per_task[], and all completions the lane counted (for the total).
+    uint64_t my_task = 0, acc = 0, lsum = 0;
+    // PT, wave-uniform: the task being walked and its last-row counts not yet added.
+    uint64_t cur_t = 0, tdirect = 0;
 
     // Wave-uniform state: the walk over the current task and the wave's claimed task range.
     typename nq::WalkFor<S>::type walk;
@@ -469,7 +482,13 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
                 uint64_t direct = 0;
                 got = walk.next(N, last, &dc, &dl, &dr, &dv, &direct);
                 wsum += direct;
+                if (PT) tdirect += direct;
                 if (got) break;
+                if (PT && tdirect) {  // the walked task is exhausted: its last-row counts
+                    if (lane == wave_leader()) atomicAdd(reinterpret_cast<unsigned long long *>(per_task + cur_t),
+                                                         static_cast<unsigned long long>(tdirect));
+                    tdirect = 0;
+                }
                 if (next >= end) {
                     // Guided claims: a share of the tasks left, 1 / (GUIDED_SHARES x waves) of them,
                     // never less than `chunk`, so single tasks near the end.  The cursor is read
@@ -502,7 +521,10 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
                 if (d0 + 1u >= N || v0 == 0) {
                     // Prefix at the last row (or full), or no legal next queen: counted here
                     // (src/n_queens.cpp:10-13 counts a full board).
-                    wsum += d0 >= N ? (c0 == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
+                    const uint32_t e = d0 >= N ? (c0 == last ? 1u : 0u) : (d0 + 1u == N ? __popc(v0) : 0u);
+                    wsum += e;
+                    if (PT && e && lane == wave_leader())
+                        atomicAdd(reinterpret_cast<unsigned long long *>(per_task + t), static_cast<unsigned long long>(e));
                     continue;
                 }
                 if (d0 + S + 1u < N && N - 1u - (d0 + S) > stack_depth) {
@@ -511,6 +533,7 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
                     continue;
                 }
                 walk.start(c0, l0, r0, v0, d0);
+                cur_t = t;
             }
             if (!got) {
                 if ((need >> lane) & 1ull) has = false;
@@ -524,13 +547,31 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
                 r = dr;
                 v = dv;
                 sp = base;
+                if (PT && my_task != cur_t) {  // the lane's roots of my_task are all counted
+                    if (acc) atomicAdd(reinterpret_cast<unsigned long long *>(per_task + my_task),
+                                       static_cast<unsigned long long>(acc));
+                    acc = 0;
+                    my_task = cur_t;
+                }
             }
         }
         const uint64_t live = __ballot(has);
         if (live == 0) break;
-        hot_loop<false, APUSH, true>(c, l, r, v, sp, cnt, wcnt, nd, live, stride);
-        wsum += wcnt;
-        wcnt = 0;
+        if (PT) {
+            uint32_t unused = 0;  // the per-wave counter of the other variant
+            hot_loop<false, APUSH, false>(c, l, r, v, sp, cnt, unused, nd, live, stride);
+            acc += cnt;
+            lsum += cnt;
+            cnt = 0;
+        } else {
+            hot_loop<false, APUSH, true>(c, l, r, v, sp, cnt, wcnt, nd, live, stride);
+            wsum += wcnt;
+            wcnt = 0;
+        }
+    }
+    if (PT) {
+        if (acc) atomicAdd(reinterpret_cast<unsigned long long *>(per_task + my_task), static_cast<unsigned long long>(acc));
+        wsum += wave_sum_u64(lsum);
     }
 
     if (trace && lane == 0) {
@@ -969,23 +1010,29 @@ const void *select_kernel(bool nodes, bool per_task, bool apush) {
     return k[nodes][per_task][apush];
 }
 
-// split_kernel for S = split rows (1..MAX_SPLIT).
-const void *select_split_kernel(int split, bool apush) {
+// split_kernel for S = split rows (1..MAX_SPLIT), with or without per-task counts.
+template <bool PT>
+const void *split_kernel_for(int split, bool apush) {
     const void *k[MAX_SPLIT][2] = {
-        {reinterpret_cast<const void *>(&split_kernel<false, 1>), reinterpret_cast<const void *>(&split_kernel<true, 1>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 2>), reinterpret_cast<const void *>(&split_kernel<true, 2>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 3>), reinterpret_cast<const void *>(&split_kernel<true, 3>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 4>), reinterpret_cast<const void *>(&split_kernel<true, 4>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 5>), reinterpret_cast<const void *>(&split_kernel<true, 5>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 6>), reinterpret_cast<const void *>(&split_kernel<true, 6>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 7>), reinterpret_cast<const void *>(&split_kernel<true, 7>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 8>), reinterpret_cast<const void *>(&split_kernel<true, 8>)}};
+        {reinterpret_cast<const void *>(&split_kernel<false, 1, PT>), reinterpret_cast<const void *>(&split_kernel<true, 1, PT>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 2, PT>), reinterpret_cast<const void *>(&split_kernel<true, 2, PT>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 3, PT>), reinterpret_cast<const void *>(&split_kernel<true, 3, PT>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 4, PT>), reinterpret_cast<const void *>(&split_kernel<true, 4, PT>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 5, PT>), reinterpret_cast<const void *>(&split_kernel<true, 5, PT>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 6, PT>), reinterpret_cast<const void *>(&split_kernel<true, 6, PT>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 7, PT>), reinterpret_cast<const void *>(&split_kernel<true, 7, PT>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 8, PT>), reinterpret_cast<const void *>(&split_kernel<true, 8, PT>)}};
     return k[split - 1][apush ? 1 : 0];
 }
 
-// The DFS kernel of a launch: split_kernel for plain totals with a device split, else dfs_kernel.
+const void *select_split_kernel(int split, bool apush, bool per_task) {
+    return per_task ? split_kernel_for<true>(split, apush) : split_kernel_for<false>(split, apush);
+}
+
+// The DFS kernel of a launch: split_kernel for totals (and per-task counts) with a device split,
+// else dfs_kernel (node counts always).
 const void *select_count_kernel(bool nodes, bool per_task, bool apush, int split) {
-    if (split > 0 && !nodes && !per_task) return select_split_kernel(split, apush);
+    if (split > 0 && !nodes) return select_split_kernel(split, apush, per_task);
     return select_kernel(nodes, per_task, apush);
 }
 
@@ -1119,10 +1166,16 @@ hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
     if (e != hipSuccess) return e;
     const size_t lds = dfs_lds_bytes(a.block, a.stack_depth + (a.apush ? 1 : 0));
     if (a.split < 0 || a.split > MAX_SPLIT) return hipErrorInvalidValue;
-    const bool split = a.split > 0 && !a.count_nodes && !a.per_task;
+    const bool split = a.split > 0 && !a.count_nodes;
     const void *k = select_count_kernel(a.count_nodes, a.per_task != nullptr, a.apush, a.split);
     e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
+    if (split && a.per_task && a.ntasks) {
+        // split_kernel adds to per_task[] (dfs_kernel writes each entry once)
+        zero_u64_kernel<<<static_cast<unsigned>(std::min<uint64_t>(2048, (a.ntasks + 255) / 256)), 256, 0, stream>>>(
+            reinterpret_cast<unsigned long long *>(a.per_task), a.ntasks);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     uint32_t N = static_cast<uint32_t>(a.N);
     uint32_t chunk = static_cast<uint32_t>(a.chunk);
     uint32_t sd = static_cast<uint32_t>(a.stack_depth);
@@ -1133,8 +1186,7 @@ hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
     uint64_t *trace = a.trace;
     const unsigned long long *ntasks_dev = a.ntasks_dev;
     void *args[] = {&tasks, &ntasks, &N, &chunk, &sd, &state, &per_task, &trace, &ntasks_dev};
-    void *split_args[] = {&tasks, &ntasks, &N, &chunk, &sd, &state, &trace, &ntasks_dev};
-    e = hipLaunchKernel(k, dim3(a.grid), dim3(a.block), split ? split_args : args, lds, stream);
+    e = hipLaunchKernel(k, dim3(a.grid), dim3(a.block), args, lds, stream);
     if (e != hipSuccess) return e;
     e = hipGetLastError();
     if (e != hipSuccess) return e;

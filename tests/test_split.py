@@ -35,13 +35,46 @@ def test_split_kernel_mixed_depths_vs_oracle():
         assert nq.count(N, t, cfg=nq.Config(split=split)).sum == want, split
 
 
-def test_split_ignored_for_per_task_and_node_counts():
-    """Per-task and node counts need one lane per task: the split does not apply to them."""
+def test_split_per_task_counts_and_nodes():
+    """Per-task counts go through the split too (split_kernel adds each root's count to its task):
+    every task's count equals the oracle's at every split depth; node counts need one lane per task
+    and ignore the split."""
     N, rows = 14, 3
     t = nq.generate(N, rows)
+    _, want = oracle.count_tasks(N, t, per_task=True)
+    for split in range(1, 9):
+        for push in (nq.NQ_PUSH_MASKED, nq.NQ_PUSH_ALWAYS):
+            r = nq.count(N, t, cfg=nq.Config(split=split, push=push), per_task=True)
+            assert np.array_equal(r.per_task, want) and r.sum == int(want.sum()), (split, push)
     r = nq.count(N, t, cfg=nq.Config(split=3), per_task=True, nodes=True)
-    assert np.array_equal(r.per_task, oracle.count_tasks(N, t, per_task=True)[1])
+    assert np.array_equal(r.per_task, want)
     assert r.nodes == oracle.count_tasks_nodes(N, t)
+
+
+def test_reference_vectors_per_task_through_the_split(ref_vectors):
+    """The reference's own per-task counts (its CPU build, tests/golden/ref_vectors.npz), counted
+    with the device split walking 1, 2, 3 and 5 rows below each task."""
+    for N, rows, total in ref_vectors["totals"]:
+        N, rows, total = int(N), int(rows), int(total)
+        tasks, counts = ref_vectors[f"N{N}_r{rows}_tasks"], ref_vectors[f"N{N}_r{rows}_counts"]
+        for split in (1, 2, 3, 5):
+            if rows + split > N - 2:
+                continue
+            r = nq.count(N, tasks, cfg=nq.Config(split=split), per_task=True)
+            assert np.array_equal(r.per_task, counts) and 2 * r.sum == total, (N, rows, split)
+
+
+def test_n21_per_task_counts(published):
+    """N=21, 6 rows: per-task counts through the automatic split sum to the total, and a seeded
+    sample of tasks matches the oracle's count of each."""
+    N, rows = 21, 6
+    t = nq.generate(N, rows)
+    r = nq.count(N, t, per_task=True)
+    assert 2 * int(r.per_task.sum()) == 2 * r.sum == published["solutions"][N]
+    rng = np.random.default_rng(21)
+    idx = np.sort(rng.choice(len(t), size=48, replace=False))
+    _, want = oracle.count_tasks(N, t[idx], per_task=True)
+    assert np.array_equal(r.per_task[idx], want)
 
 
 def test_split_above_the_limit_is_an_argument_error():

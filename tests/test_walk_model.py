@@ -89,3 +89,36 @@ def test_walk_model_flags_a_stack_too_small(model, tmp_path):
     total, flags, roots, claims, frames, done = run_model(model, tmp_path, N, tasks, split, 4, 3,
                                                           N - 1 - (rows + split) - 1)
     assert flags == 1 and roots == 0 and claims and done
+
+
+@pytest.mark.parametrize("lanes", [False, True])
+def test_walk_model_per_task_counts(model, tmp_path, lanes):
+    """split_kernel's per-task counts (PT): lanes add their roots' counts per task when they move
+    to another task and at exit, the walk's last-row counts go to their task when it moves on, and
+    edge tasks' when claimed; every task's count equals the oracle's, for plain lists and a list of
+    mixed depths, at every split depth and several claim sizes and wave counts."""
+    cases = []
+    for N, rows in ((9, 2), (11, 3), (12, 4), (13, 5)):
+        cases.append((N, oracle.gen_tasks(N, rows), rows))
+    N = 11
+    rng = np.random.default_rng(7)
+    parts = [oracle.gen_tasks(N, d) for d in range(1, N + 1)]
+    mixed = np.concatenate([p[rng.choice(len(p), min(len(p), 30), replace=False)] for p in parts if len(p)])
+    mixed = np.concatenate([mixed, np.array([[0b11, 0b1100, 0], [0, 0, 0]], dtype=np.uint32)])
+    rng.shuffle(mixed)
+    cases.append((N, mixed, 0))
+    out = tmp_path / "per_task.bin"
+    for N, tasks, d0 in cases:
+        _, want = oracle.count_tasks(N, tasks, per_task=True)
+        for split in range(1, 9):
+            if d0 + split > N - 2:
+                continue
+            for chunk, waves in ((1, 1), (3, 7)):
+                f = tmp_path / "tasks.bin"
+                np.ascontiguousarray(tasks, dtype=np.uint32).tofile(f)
+                stack = max(1, N - 1 - (d0 + split))
+                res = subprocess.check_output([model, str(N), str(split), str(chunk), str(waves), str(stack), str(f),
+                                               "lanes" if lanes else "regs", str(out)], timeout=120).split()
+                assert int(res[1]) == 0 and int(res[0]) == int(want.sum()), (N, split, chunk, waves)
+                got = np.fromfile(out, dtype=np.uint64)
+                assert np.array_equal(got, want), (N, d0, split, chunk, waves)

@@ -11,6 +11,10 @@
 // The walk is TaskWalk (scalar frames, split <= 4) or LaneWalk (frames in lanes, split 5..8), as
 // split_kernel picks them; a seventh argument `lanes` runs LaneWalk at every split.
 //
+// Per-task counts follow split_kernel's PT protocol: a lane adds the counts of its current task's
+// roots when it takes a root of another task and at exit, the walk's last-row counts go to their
+// task when the walk moves on, edge tasks' counts when they are claimed.
+//
 // Checks, reported on stdout as one line "total flags roots claims_ok frames_ok terminated
 // cursor_claims":
 //   * every task index is claimed exactly once;
@@ -18,7 +22,8 @@
 //   * every wave terminates.
 // The total is compared with the oracle by tests/test_walk_model.py.
 //
-//   walk_model N split chunk waves stack_depth taskfile [lanes]     (taskfile: uint32 triples)
+//   walk_model N split chunk waves stack_depth taskfile [lanes|regs [per_task_out]]
+//   (taskfile: uint32 triples; per_task_out: the per-task counts as uint64, in task order)
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -53,6 +58,7 @@ struct Lane {
     bool has = true;
     bool parked = true;  // v == 0 in the kernel: no root, or the root is exhausted
     uint64_t work = 0;
+    uint64_t my_task = 0, acc = 0;  // per-task counts: the kernel's lane accumulator (PT)
 };
 
 struct Model {
@@ -63,6 +69,8 @@ struct Model {
     std::vector<int> claimed;
     uint64_t total = 0, flags = 0, roots = 0;
     bool frames_ok = true;
+    std::vector<uint64_t> per_task;  // split_kernel<..., PT>'s per-task counts, as it adds them
+    const char *per_task_out = nullptr;
 };
 
 template <typename W>
@@ -73,6 +81,7 @@ struct Wave {
     uint64_t next = 0, end = 0;
     bool drained = false, finished = false;
     uint64_t wsum = 0;
+    uint64_t cur_t = 0, tdirect = 0;  // PT: the task being walked, its last-row counts not yet added
 
     // One pass of the kernel's outer loop: refill, then trips until some lane parks.
     void step(Model &m) {
@@ -86,7 +95,12 @@ struct Wave {
                 uint64_t direct = 0;
                 got = walk.next(m.N, m.last, &dc, &dl, &dr, &dv, &direct);
                 wsum += direct;
+                tdirect += direct;
                 if (got) break;
+                if (tdirect) {  // the walked task is exhausted: its last-row counts
+                    m.per_task[cur_t] += tdirect;
+                    tdirect = 0;
+                }
                 if (next >= end) {
                     // split_kernel's guided claim: 1 / (GUIDED_SHARES = 4 x waves) of the tasks
                     // left at the current cursor, at least `chunk`.
@@ -108,7 +122,9 @@ struct Wave {
                 const uint32_t v0 = m.last & ~(c0 | l0 | r0);
                 const uint32_t d0 = __builtin_popcount(c0);
                 if (d0 + 1 >= m.N || v0 == 0) {
-                    wsum += d0 >= m.N ? (c0 == m.last ? 1u : 0u) : (d0 + 1 == m.N ? __builtin_popcount(v0) : 0u);
+                    const uint32_t e = d0 >= m.N ? (c0 == m.last ? 1u : 0u) : (d0 + 1 == m.N ? __builtin_popcount(v0) : 0u);
+                    wsum += e;
+                    m.per_task[t] += e;
                     continue;
                 }
                 if (d0 + S + 1 < m.N && m.N - 1 - (d0 + S) > m.stack_depth) {
@@ -116,6 +132,7 @@ struct Wave {
                     continue;
                 }
                 walk.start(c0, l0, r0, v0, d0);
+                cur_t = t;
             }
             if (!got) {
                 for (int i = 0; i < 64; ++i)
@@ -133,6 +150,12 @@ struct Wave {
             wsum += sub.count;
             lane[sel].parked = false;
             lane[sel].work = sub.expansions;
+            if (lane[sel].my_task != cur_t) {  // the lane's roots of my_task are all counted
+                m.per_task[lane[sel].my_task] += lane[sel].acc;
+                lane[sel].acc = 0;
+                lane[sel].my_task = cur_t;
+            }
+            lane[sel].acc += sub.count;  // folded from the lane's counter while it runs the root
             m.roots++;
         }
         bool live = false;
@@ -140,6 +163,7 @@ struct Wave {
         if (!live) {
             finished = true;
             m.total += wsum;
+            for (auto &ln : lane) m.per_task[ln.my_task] += ln.acc;  // the kernel's exit flush
             return;
         }
         // Trips of 16 iterations until some live lane is parked at a trip's end.
@@ -181,17 +205,24 @@ int run(Model &m, int nwaves) {
     printf("%llu %llu %llu %d %d %d %llu\n", (unsigned long long)m.total, (unsigned long long)m.flags,
            (unsigned long long)m.roots, claims_ok ? 1 : 0, m.frames_ok ? 1 : 0, all ? 1 : 0,
            (unsigned long long)m.claims);
+    if (m.per_task_out) {
+        FILE *f = fopen(m.per_task_out, "wb");
+        if (f) {
+            fwrite(m.per_task.data(), sizeof(uint64_t), m.per_task.size(), f);
+            fclose(f);
+        }
+    }
     return 0;
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc != 7 && argc != 8) {
-        fprintf(stderr, "usage: %s N split chunk waves stack_depth taskfile [lanes]\n", argv[0]);
+    if (argc < 7 || argc > 9) {
+        fprintf(stderr, "usage: %s N split chunk waves stack_depth taskfile [lanes|regs [per_task_out]]\n", argv[0]);
         return 2;
     }
-    const bool lanes = argc == 8 && !strcmp(argv[7], "lanes");
+    const bool lanes = argc >= 8 && !strcmp(argv[7], "lanes");
     Model m;
     m.N = static_cast<uint32_t>(atoi(argv[1]));
     const int split = atoi(argv[2]);
@@ -208,6 +239,8 @@ int main(int argc, char **argv) {
     m.tasks = &tasks;
     m.ntasks = tasks.size() / 3;
     m.claimed.assign(m.ntasks, 0);
+    m.per_task.assign(m.ntasks, 0);
+    if (argc == 9) m.per_task_out = argv[8];
     if (lanes) switch (split) {
             case 1: return run<nq::LaneWalk<1>>(m, nwaves);
             case 2: return run<nq::LaneWalk<2>>(m, nwaves);
