@@ -125,6 +125,40 @@ def test_count_device_split_is_captured_in_a_graph(published):
         assert int(junk) == (1 << 20) * ((1 << 20) - 1) // 2
 
 
+def test_count_device_per_task_split_is_captured_in_a_graph():
+    """Per-task counts through the split, captured in a graph: the per-task array is cleared by a
+    kernel (no memset node) before split_kernel adds to it, so every replay, with eager work and a
+    poisoned array between replays, gives the eager per-task counts."""
+    import torch
+    N, rows = 16, 4
+    tasks = nq.generate(N, rows)
+    dev = torch.device("cuda:0")
+    d_tasks = torch.from_numpy(tasks.view(np.int32).reshape(-1)).to(dev)
+    d_res = torch.zeros(3, dtype=torch.int64, device=dev)
+    d_pt = torch.zeros(len(tasks), dtype=torch.int64, device=dev)
+    cfg = nq.Config(split=3, stack_depth=N - 1 - rows)
+    s = torch.cuda.Stream(device=dev)
+    nq.count_device(N, d_tasks.data_ptr(), len(tasks), d_res.data_ptr(), d_per_task=d_pt.data_ptr(), cfg=cfg,
+                    stream=s.cuda_stream)
+    s.synchronize()
+    eager = d_pt.cpu().numpy().view(np.uint64).copy()
+    assert np.array_equal(eager, oracle.count_tasks(N, tasks, per_task=True)[1])
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+        nq.count_device(N, d_tasks.data_ptr(), len(tasks), d_res.data_ptr(), d_per_task=d_pt.data_ptr(), cfg=cfg,
+                        stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for i in range(4):
+        d_pt.fill_(7)  # the replay must clear what a previous use left
+        junk = torch.arange(1 << 20, device=dev).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(d_pt.cpu().numpy().view(np.uint64), eager), i
+        assert int(junk) == (1 << 20) * ((1 << 20) - 1) // 2
+
+
 def test_reference_single_gpu_setting_and_automatic_splits(published, monkeypatch):
     """The reference's advice for one GPU (README: N <= 22 with 5 rows pre-placed under
     configuration3) through the CLI at N=20, and N=21 with 5 and 4 rows, where the automatic split
