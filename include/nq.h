@@ -47,8 +47,8 @@ typedef struct nq_config {
     int split;        /* rows the DFS kernel walks below each task before its LDS DFS     */
                       /* (0..8; -1 = automatic): a wave claims a whole task and hands the   */
                       /* nodes `split` rows below it to its lanes as DFS roots, in registers */
-                      /* (no expanded list in memory).  Totals and per-task counts; ignored */
-                      /* when node counts are requested.                                    */
+                      /* (no expanded list in memory).  Totals, per-task or node counts;    */
+                      /* ignored when per-task and node counts are both requested.          */
     int push;         /* DFS push form: NQ_PUSH_AUTO (0) takes NQ_PUSH_ALWAYS when its spare */
                       /* stack level costs no resident waves, else NQ_PUSH_MASKED.  Both    */
                       /* give identical counts.                                              */
@@ -102,8 +102,7 @@ int nq_config_default(int N, int d0, nq_config *cfg);
 int nq_solve_config(int N, int rows, nq_config *cfg);
 
 /* The launch configuration nq_count takes on the current device for ntasks tasks whose
- * shallowest prefix depth is d0 (after any device split; totals, no node counts): cfg's zero
- * fields resolved, i.e. the workgroup size, the push form (never
+ * shallowest prefix depth is d0 (after any device split; totals): cfg's zero fields resolved, i.e. the workgroup size, the push form (never
  * NQ_PUSH_AUTO in *out) and the persistent grid.  No reference counterpart: it reports what the
  * reference fixes at compile time (src/macro.h:25-35, src/n_queens_cuda.cu:164-167). */
 int nq_config_resolve(int N, int d0, uint64_t ntasks, const nq_config *cfg, nq_config *out);

@@ -146,7 +146,7 @@ static int resolve_config(int cus, int N, int d0, uint64_t ntasks, const nq_conf
     }
     // With the split a wave claims whole tasks: one at a time unless told otherwise, so that no
     // wave holds claimed tasks back while others run dry at the end.
-    if (split > 0 && !nodes && !(in && in->chunk)) c.chunk = 1;
+    if (split > 0 && !(nodes && per_task) && !(in && in->chunk)) c.chunk = 1;
     int rc = check_cfg_shape(c);
     if (rc) return rc;
     if (c.stack_depth < frames_needed(N, d0)) {
@@ -192,7 +192,7 @@ static int resolve_config(int cus, int N, int d0, uint64_t ntasks, const nq_conf
     if (c.grid == 0) c.grid = cus * bpc;
     // Workgroups the tasks can keep busy: one lane per task, or with the split one wave per task
     // (a wave walks a claimed task's roots on its 64 lanes).
-    const uint64_t per_wg = split > 0 && !nodes ? static_cast<uint64_t>(c.block / 64) : c.block;
+    const uint64_t per_wg = split > 0 && !(nodes && per_task) ? static_cast<uint64_t>(c.block / 64) : c.block;
     const uint64_t wgs_needed = (ntasks + per_wg - 1) / per_wg;
     if (static_cast<uint64_t>(c.grid) > wgs_needed) c.grid = static_cast<int>(std::max<uint64_t>(1, wgs_needed));
     c.split = split;
@@ -217,7 +217,7 @@ int plan_count(int cus, int N, int d0, uint64_t ntasks, const nq_config *cfg, bo
     nq_config in = cfg ? *cfg : nq_config{};
     if (!cfg) in.split = -1;
     int split = 0;
-    if (!nodes) {
+    if (!(nodes && per_task)) {
         int rc = effective_split(N, d0, &in, &split);
         if (rc) return rc;
     }
@@ -580,7 +580,7 @@ int nq_count_device(int N, const uint32_t *d_tasks, uint64_t ntasks, const nq_co
     nq_config in = cfg ? *cfg : nq_config{};
     if (!cfg) in.split = -1;
     int split = 0;
-    if (!count_nodes) {
+    if (!(count_nodes && d_per_task)) {
         rc = effective_split(N, d0, &in, &split);
         if (rc) return rc;
     }
@@ -613,7 +613,7 @@ int nq_count(int N, const uint32_t *tasks, uint64_t ntasks, const nq_config *cfg
     nq_config in = cfg ? *cfg : nq_config{};
     if (!cfg) in.split = -1;
     int split = 0;
-    if (!nodes) {
+    if (!(nodes && per_task)) {
         rc = effective_split(N, d0, &in, &split);
         if (rc) return rc;
     }

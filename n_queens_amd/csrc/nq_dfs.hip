@@ -431,17 +431,20 @@ __device__ __forceinline__ uint32_t wave_leader() {
 // subtree stays on the wave that claimed it (spread over its 64 lanes).  The DFS is
 // dfs_kernel's hot loop with the per-wave board counter.
 //
-// PT: per-task counts as well (the reference kernel's partial_sum, src/n_queens_cuda.cu:79), into
-// per_task[], zeroed before the launch.  The hot loop then counts per lane; a lane keeps the task
-// of its current root and the completions of that task's roots it has run, and adds them to
-// per_task[] with one atomic when it takes a root of another task (a wave walks its tasks one after
-// another, so a lane's roots come in task order) and at exit.  The walk's last-row counts go to
-// their task when the wave moves on to the next task, edge tasks' counts when they are claimed.
-template <bool APUSH, int S, bool PT>
+// MODE 0: totals.  MODE 1 (PT): per-task counts as well (the reference kernel's partial_sum,
+// src/n_queens_cuda.cu:79), into per_task[], zeroed before the launch.  The hot loop then counts
+// per lane; a lane keeps the task of its current root and the completions of that task's roots it
+// has run, and adds them to per_task[] with one atomic when it takes a root of another task (a
+// wave walks its tasks one after another, so a lane's roots come in task order) and at exit.  The
+// walk's last-row counts go to their task when the wave moves on to the next task, edge tasks'
+// counts when they are claimed.  MODE 2 (NODES): DFS node counts as well: the walk's placements
+// plus, per lane, the hot loop's expansions less its completed boards (dfs_kernel's count).
+template <bool APUSH, int S, int MODE>
 __global__ void __launch_bounds__(MAX_BLOCK)
 split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, uint32_t chunk, uint32_t stack_depth,
              DevState *__restrict__ st, uint64_t *__restrict__ per_task, uint64_t *__restrict__ trace,
              const unsigned long long *__restrict__ ntasks_dev) {
+    constexpr bool PT = MODE == 1, NODES = MODE == 2, LANE_COUNT = MODE != 0;
     if (ntasks_dev) ntasks = min(ntasks, static_cast<uint64_t>(*ntasks_dev));
     extern __shared__ uint4 lds[];  // lds_layout(): dead frame, stack, wq (unused), red
     if (!lds_base_ok(lds, st)) return;
@@ -461,8 +464,9 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
     uint64_t wsum = 0;                    // the wave's count: folded wcnt + last-row and edge counts
     bool has = true;                      // lane still has (or may get) a DFS root
     // PT, per lane: the task of the current root, the completions of that task's roots not yet
-    // added to per_task[], and all completions the lane counted (for the total).
-    uint64_t my_task = 0, acc = 0, lsum = 0;
+    // added to per_task[], and (PT and NODES) all completions the lane counted, for the total;
+    // NODES, per lane: the DFS nodes of its roots.
+    uint64_t my_task = 0, acc = 0, lsum = 0, nodes = 0;
     // PT, wave-uniform: the task being walked and its last-row counts not yet added.
     uint64_t cur_t = 0, tdirect = 0;
 
@@ -557,22 +561,24 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
         }
         const uint64_t live = __ballot(has);
         if (live == 0) break;
-        if (PT) {
+        if (LANE_COUNT) {
             uint32_t unused = 0;  // the per-wave counter of the other variant
-            hot_loop<false, APUSH, false>(c, l, r, v, sp, cnt, unused, nd, live, stride);
-            acc += cnt;
+            hot_loop<NODES, APUSH, false>(c, l, r, v, sp, cnt, unused, nd, live, stride);
+            if (PT) acc += cnt;
+            if (NODES) nodes += static_cast<uint64_t>(nd) - cnt;
             lsum += cnt;
             cnt = 0;
+            nd = 0;
         } else {
             hot_loop<false, APUSH, true>(c, l, r, v, sp, cnt, wcnt, nd, live, stride);
             wsum += wcnt;
             wcnt = 0;
         }
     }
-    if (PT) {
-        if (acc) atomicAdd(reinterpret_cast<unsigned long long *>(per_task + my_task), static_cast<unsigned long long>(acc));
-        wsum += wave_sum_u64(lsum);
-    }
+    if (PT && acc) atomicAdd(reinterpret_cast<unsigned long long *>(per_task + my_task), static_cast<unsigned long long>(acc));
+    if (LANE_COUNT) wsum += wave_sum_u64(lsum);
+    uint64_t wnodes = 0;
+    if (NODES) wnodes = wave_sum_u64(nodes) + walk.placed;
 
     if (trace && lane == 0) {
         const uint64_t w = static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + wave;
@@ -581,13 +587,17 @@ split_kernel(const uint32_t *__restrict__ tasks, uint64_t ntasks, uint32_t N, ui
     }
     if (lane == 0) {
         red[wave][0] = wsum;
-        red[wave][1] = 0;
+        red[wave][1] = wnodes;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long s = 0;
-        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) s += red[w][0];
+        unsigned long long s = 0, n = 0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
+            s += red[w][0];
+            n += red[w][1];
+        }
         atomicAdd(&st->total, s);
+        if (NODES) atomicAdd(&st->nodes, n);
     }
 }
 
@@ -1010,29 +1020,30 @@ const void *select_kernel(bool nodes, bool per_task, bool apush) {
     return k[nodes][per_task][apush];
 }
 
-// split_kernel for S = split rows (1..MAX_SPLIT), with or without per-task counts.
-template <bool PT>
+// split_kernel for S = split rows (1..MAX_SPLIT) and a counting mode (0 totals, 1 per-task, 2 nodes).
+template <int MODE>
 const void *split_kernel_for(int split, bool apush) {
     const void *k[MAX_SPLIT][2] = {
-        {reinterpret_cast<const void *>(&split_kernel<false, 1, PT>), reinterpret_cast<const void *>(&split_kernel<true, 1, PT>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 2, PT>), reinterpret_cast<const void *>(&split_kernel<true, 2, PT>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 3, PT>), reinterpret_cast<const void *>(&split_kernel<true, 3, PT>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 4, PT>), reinterpret_cast<const void *>(&split_kernel<true, 4, PT>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 5, PT>), reinterpret_cast<const void *>(&split_kernel<true, 5, PT>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 6, PT>), reinterpret_cast<const void *>(&split_kernel<true, 6, PT>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 7, PT>), reinterpret_cast<const void *>(&split_kernel<true, 7, PT>)},
-        {reinterpret_cast<const void *>(&split_kernel<false, 8, PT>), reinterpret_cast<const void *>(&split_kernel<true, 8, PT>)}};
+        {reinterpret_cast<const void *>(&split_kernel<false, 1, MODE>), reinterpret_cast<const void *>(&split_kernel<true, 1, MODE>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 2, MODE>), reinterpret_cast<const void *>(&split_kernel<true, 2, MODE>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 3, MODE>), reinterpret_cast<const void *>(&split_kernel<true, 3, MODE>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 4, MODE>), reinterpret_cast<const void *>(&split_kernel<true, 4, MODE>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 5, MODE>), reinterpret_cast<const void *>(&split_kernel<true, 5, MODE>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 6, MODE>), reinterpret_cast<const void *>(&split_kernel<true, 6, MODE>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 7, MODE>), reinterpret_cast<const void *>(&split_kernel<true, 7, MODE>)},
+        {reinterpret_cast<const void *>(&split_kernel<false, 8, MODE>), reinterpret_cast<const void *>(&split_kernel<true, 8, MODE>)}};
     return k[split - 1][apush ? 1 : 0];
 }
 
-const void *select_split_kernel(int split, bool apush, bool per_task) {
-    return per_task ? split_kernel_for<true>(split, apush) : split_kernel_for<false>(split, apush);
+const void *select_split_kernel(int split, bool apush, bool per_task, bool nodes) {
+    if (nodes) return split_kernel_for<2>(split, apush);
+    return per_task ? split_kernel_for<1>(split, apush) : split_kernel_for<0>(split, apush);
 }
 
-// The DFS kernel of a launch: split_kernel for totals (and per-task counts) with a device split,
-// else dfs_kernel (node counts always).
+// The DFS kernel of a launch: split_kernel with a device split (totals, per-task counts or node
+// counts), else dfs_kernel (also per-task and node counts together).
 const void *select_count_kernel(bool nodes, bool per_task, bool apush, int split) {
-    if (split > 0 && !nodes) return select_split_kernel(split, apush, per_task);
+    if (split > 0 && !(nodes && per_task)) return select_split_kernel(split, apush, per_task, nodes);
     return select_kernel(nodes, per_task, apush);
 }
 
@@ -1166,7 +1177,7 @@ hipError_t launch_count(const LaunchArgs &a, hipStream_t stream) {
     if (e != hipSuccess) return e;
     const size_t lds = dfs_lds_bytes(a.block, a.stack_depth + (a.apush ? 1 : 0));
     if (a.split < 0 || a.split > MAX_SPLIT) return hipErrorInvalidValue;
-    const bool split = a.split > 0 && !a.count_nodes;
+    const bool split = a.split > 0 && !(a.count_nodes && a.per_task);
     const void *k = select_count_kernel(a.count_nodes, a.per_task != nullptr, a.apush, a.split);
     e = allow_lds(k, lds);
     if (e != hipSuccess) return e;

@@ -42,8 +42,8 @@ struct LaunchArgs {
     const unsigned long long *ntasks_dev = nullptr;  // nullable: task count on the device (<= ntasks)
     const unsigned long long *xflags = nullptr;      // nullable: expansion flags folded into result[2]
     bool apush = false;  // unconditional push (the launch allocates stack_depth + 1 levels)
-    // Rows enumerated in registers below every task before its DFS (0..MAX_SPLIT; totals and
-    // per-task counts, ignored with count_nodes).  stack_depth then bounds the tasks' roots.
+    // Rows enumerated in registers below every task before its DFS (0..MAX_SPLIT; totals,
+    // per-task or node counts, ignored with both).  stack_depth then bounds the tasks' roots.
     int split = 0;
 };
 

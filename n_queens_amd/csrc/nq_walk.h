@@ -10,7 +10,9 @@
 // DFS root.  A node met on the way that sits on the last row (one free column left) is not
 // handed out: its count, the popcount of its candidates, is returned through *direct, as the
 // reference kernel counts the last row (src/n_queens_cuda.cu:59-67).  The counts of all
-// handed-out nodes plus *direct sum to the task's count.
+// handed-out nodes plus *direct sum to the task's count.  `placed` counts the queens the walk
+// places (each one a DFS node in the sense of src/n_queens_cuda.cu:39-72: a placement from a
+// frame above the last row), so that a node count is `placed` plus the roots' own nodes.
 //
 // The header is plain C++ so that tests/walk_model.cpp runs the same code on the CPU.
 #pragma once
@@ -41,7 +43,8 @@ struct TaskWalk {
     // the candidates not yet taken.  Frames 0..top are valid.
     uint32_t c[S], l[S], r[S], v[S];
     int top = -1;
-    uint32_t d0 = 0;  // rows the task has placed
+    uint32_t d0 = 0;      // rows the task has placed
+    uint64_t placed = 0;  // queens placed by the walk so far, over every task (DFS nodes)
 
     // A task at depth d (d <= N - 2) whose candidate set v0 is not empty.
     NQ_HD void start(uint32_t c0, uint32_t l0, uint32_t r0, uint32_t v0, uint32_t d) {
@@ -75,6 +78,7 @@ struct TaskWalk {
                     fl = l[k];
                     fr = r[k];
                 }
+            ++placed;
             const uint32_t cc = fc | p, cl = (fl | p) << 1, cr = (fr | p) >> 1;
             const uint32_t cv = last & ~(cc | cl | cr);
             if (cv == 0) continue;
@@ -136,6 +140,7 @@ struct LaneWalk {
     LaneWord c, l, r, v;
     int top = -1;
     uint32_t d0 = 0;
+    uint64_t placed = 0;  // queens placed by the walk so far, over every task (DFS nodes)
 
     NQ_HD void start(uint32_t c0, uint32_t l0, uint32_t r0, uint32_t v0, uint32_t d) {
         c.x[0] = l.x[0] = r.x[0] = v.x[0] = 0;
@@ -155,6 +160,7 @@ struct LaneWalk {
             if (top < 0) return false;
             const uint32_t p = fv & (0u - fv);
             v.set(top, fv ^ p);
+            ++placed;
             const uint32_t cc = c.get(top) | p, cl = (l.get(top) | p) << 1, cr = (r.get(top) | p) >> 1;
             const uint32_t cv = last & ~(cc | cl | cr);
             if (cv == 0) continue;

@@ -36,19 +36,23 @@ def test_split_kernel_mixed_depths_vs_oracle():
 
 
 def test_split_per_task_counts_and_nodes():
-    """Per-task counts go through the split too (split_kernel adds each root's count to its task):
-    every task's count equals the oracle's at every split depth; node counts need one lane per task
-    and ignore the split."""
+    """Per-task counts and node counts go through the split too (split_kernel adds each root's
+    count to its task; its node count is the walk's placements plus each root's expansions less
+    its boards): every task's count and the node count equal the oracle's at every split depth;
+    per-task and node counts asked for together take one lane per task (dfs_kernel)."""
     N, rows = 14, 3
     t = nq.generate(N, rows)
     _, want = oracle.count_tasks(N, t, per_task=True)
+    want_nodes = oracle.count_tasks_nodes(N, t)
     for split in range(1, 9):
         for push in (nq.NQ_PUSH_MASKED, nq.NQ_PUSH_ALWAYS):
             r = nq.count(N, t, cfg=nq.Config(split=split, push=push), per_task=True)
             assert np.array_equal(r.per_task, want) and r.sum == int(want.sum()), (split, push)
+            r = nq.count(N, t, cfg=nq.Config(split=split, push=push), nodes=True)
+            assert r.nodes == want_nodes and r.sum == int(want.sum()), (split, push, r.nodes, want_nodes)
     r = nq.count(N, t, cfg=nq.Config(split=3), per_task=True, nodes=True)
     assert np.array_equal(r.per_task, want)
-    assert r.nodes == oracle.count_tasks_nodes(N, t)
+    assert r.nodes == want_nodes
 
 
 def test_reference_vectors_per_task_through_the_split(ref_vectors):

@@ -27,8 +27,9 @@ def run_model(model, tmp_path, N, tasks, split, chunk, waves, stack, lanes=False
     np.ascontiguousarray(tasks, dtype=np.uint32).tofile(f)
     out = subprocess.check_output([model, str(N), str(split), str(chunk), str(waves), str(stack), str(f)]
                                   + (["lanes"] if lanes else []), timeout=120).split()
-    total, flags, roots, claims_ok, frames_ok, done, cursor_claims = (int(x) for x in out)
+    total, flags, roots, claims_ok, frames_ok, done, cursor_claims, nodes = (int(x) for x in out)
     run_model.cursor_claims = cursor_claims
+    run_model.nodes = nodes
     return total, flags, roots, claims_ok == 1, frames_ok == 1, done == 1
 
 
@@ -122,3 +123,27 @@ def test_walk_model_per_task_counts(model, tmp_path, lanes):
                 assert int(res[1]) == 0 and int(res[0]) == int(want.sum()), (N, split, chunk, waves)
                 got = np.fromfile(out, dtype=np.uint64)
                 assert np.array_equal(got, want), (N, d0, split, chunk, waves)
+
+
+@pytest.mark.parametrize("lanes", [False, True])
+def test_walk_model_node_counts(model, tmp_path, lanes):
+    """split_kernel's node count (MODE 2): the placements the walk makes plus, for every root, the
+    hot loop's expansions less its completed boards, equals the oracle's kernel-sense node count
+    (placements at depths rows+1..N-1) at every split depth, for plain and mixed-depth lists."""
+    N = 11
+    rng = np.random.default_rng(11)
+    parts = [oracle.gen_tasks(N, d) for d in range(1, N + 1)]
+    mixed = np.concatenate([p[rng.choice(len(p), min(len(p), 30), replace=False)] for p in parts if len(p)])
+    cases = [(9, oracle.gen_tasks(9, 2), 2), (12, oracle.gen_tasks(12, 3), 3), (13, oracle.gen_tasks(13, 5), 5),
+             (N, mixed, 1)]
+    for n, tasks, d0 in cases:
+        want = oracle.count_tasks_nodes(n, tasks)
+        for split in range(1, 9):
+            if d0 + split > n - 2:
+                continue
+            stack = max(1, n - 1 - (d0 + split))
+            f = tmp_path / "tasks.bin"
+            np.ascontiguousarray(tasks, dtype=np.uint32).tofile(f)
+            out = subprocess.check_output([model, str(n), str(split), "2", "5", str(stack), str(f),
+                                           "lanes" if lanes else "regs"], timeout=120).split()
+            assert int(out[1]) == 0 and int(out[7]) == want, (n, d0, split, int(out[7]), want)

@@ -16,7 +16,8 @@
 // task when the walk moves on, edge tasks' counts when they are claimed.
 //
 // Checks, reported on stdout as one line "total flags roots claims_ok frames_ok terminated
-// cursor_claims":
+// cursor_claims nodes" (nodes: split_kernel's node count, the walks' placements plus each root's
+// expansions less its completed boards):
 //   * every task index is claimed exactly once;
 //   * every handed-out root needs at most stack_depth frames (N - 1 - its depth);
 //   * every wave terminates.
@@ -68,6 +69,7 @@ struct Model {
     uint64_t cursor = 0, nwaves = 1, claims = 0;
     std::vector<int> claimed;
     uint64_t total = 0, flags = 0, roots = 0;
+    uint64_t nodes = 0;  // split_kernel's node count: the walks' placements + each root's own nodes
     bool frames_ok = true;
     std::vector<uint64_t> per_task;  // split_kernel<..., PT>'s per-task counts, as it adds them
     const char *per_task_out = nullptr;
@@ -148,6 +150,7 @@ struct Wave {
             Sub sub;
             subtree(m.last, dc, dl, dr, &sub);
             wsum += sub.count;
+            m.nodes += sub.expansions - sub.count;  // the hot loop's expansions less its boards
             lane[sel].parked = false;
             lane[sel].work = sub.expansions;
             if (lane[sel].my_task != cur_t) {  // the lane's roots of my_task are all counted
@@ -163,6 +166,7 @@ struct Wave {
         if (!live) {
             finished = true;
             m.total += wsum;
+            m.nodes += walk.placed;
             for (auto &ln : lane) m.per_task[ln.my_task] += ln.acc;  // the kernel's exit flush
             return;
         }
@@ -202,9 +206,9 @@ int run(Model &m, int nwaves) {
     }
     bool claims_ok = true;
     for (int c : m.claimed) claims_ok &= c == 1;
-    printf("%llu %llu %llu %d %d %d %llu\n", (unsigned long long)m.total, (unsigned long long)m.flags,
+    printf("%llu %llu %llu %d %d %d %llu %llu\n", (unsigned long long)m.total, (unsigned long long)m.flags,
            (unsigned long long)m.roots, claims_ok ? 1 : 0, m.frames_ok ? 1 : 0, all ? 1 : 0,
-           (unsigned long long)m.claims);
+           (unsigned long long)m.claims, (unsigned long long)m.nodes);
     if (m.per_task_out) {
         FILE *f = fopen(m.per_task_out, "wb");
         if (f) {
